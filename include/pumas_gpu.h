@@ -1,0 +1,316 @@
+/*
+ * libpumas_hip: MI355X (gfx950) transport of batches of independent muon
+ * histories, behind a plain C ABI.
+ *
+ * The device path replaces, for a whole batch at once, the per-history call
+ *     pumas_context_transport(context, state, event, media)
+ * of the reference library (include/pumas.h:1046-1048, src/pumas.c:3114-3251).
+ * What the reference takes as host callbacks becomes data:
+ *   - `context->medium` / `pumas_medium.locals`  -> a device geometry
+ *     (struct pumas_gpu_geometry_desc), media identified by index;
+ *   - `context->random`                           -> a counter based
+ *     Philox4x32-10 stream keyed by (seed, history id);
+ *   - `pumas_physics` (opaque, src/pumas.c:660-806) -> a PTAB table file
+ *     exported from it once and uploaded once.
+ * The state, mode, event, limit and return-code surface is the reference's,
+ * with identical layouts and enum values, so that a `struct pumas_state`
+ * array filled for the reference can be handed over unchanged.
+ */
+#ifndef PUMAS_GPU_H
+#define PUMAS_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef pumas_h
+/*
+ * Types shared with the reference public header. When include/pumas.h of the
+ * reference is included first (it defines `pumas_h`), its own definitions are
+ * used instead; layouts and values are identical.
+ */
+
+/* Transport modes, include/pumas.h:90-140. */
+enum pumas_mode {
+        PUMAS_MODE_DISABLED = -1,
+        PUMAS_MODE_CSDA = 0,
+        PUMAS_MODE_MIXED = 1,
+        PUMAS_MODE_STRAGGLED = 2,
+        PUMAS_MODE_WEIGHTED = 0,
+        PUMAS_MODE_RANDOMISED = 1,
+        PUMAS_MODE_FORWARD = 0,
+        PUMAS_MODE_BACKWARD = 1
+};
+
+/* Return codes, include/pumas.h:143-196. */
+enum pumas_return {
+        PUMAS_RETURN_SUCCESS = 0,
+        PUMAS_RETURN_ACCURACY_ERROR,
+        PUMAS_RETURN_END_OF_FILE,
+        PUMAS_RETURN_DECAY_ERROR,
+        PUMAS_RETURN_DENSITY_ERROR,
+        PUMAS_RETURN_DIRECTION_ERROR,
+        PUMAS_RETURN_INCOMPLETE_FILE,
+        PUMAS_RETURN_INDEX_ERROR,
+        PUMAS_RETURN_PHYSICS_ERROR,
+        PUMAS_RETURN_INTERNAL_ERROR,
+        PUMAS_RETURN_IO_ERROR,
+        PUMAS_RETURN_FORMAT_ERROR,
+        PUMAS_RETURN_MEDIUM_ERROR,
+        PUMAS_RETURN_MEMORY_ERROR,
+        PUMAS_RETURN_MODEL_ERROR,
+        PUMAS_RETURN_MISSING_LIMIT,
+        PUMAS_RETURN_MISSING_RANDOM,
+        PUMAS_RETURN_PATH_ERROR,
+        PUMAS_RETURN_RAISE_ERROR,
+        PUMAS_RETURN_TOO_LONG,
+        PUMAS_RETURN_UNDEFINED_MDF,
+        PUMAS_RETURN_UNKNOWN_ELEMENT,
+        PUMAS_RETURN_UNKNOWN_MATERIAL,
+        PUMAS_RETURN_UNKNOWN_PARTICLE,
+        PUMAS_RETURN_VALUE_ERROR,
+        PUMAS_N_RETURNS
+};
+
+/* Transport events, include/pumas.h:199-236. */
+enum pumas_event {
+        PUMAS_EVENT_NONE = 0,
+        PUMAS_EVENT_LIMIT_ENERGY = 1,
+        PUMAS_EVENT_LIMIT_DISTANCE = 2,
+        PUMAS_EVENT_LIMIT_GRAMMAGE = 4,
+        PUMAS_EVENT_LIMIT_TIME = 8,
+        PUMAS_EVENT_LIMIT = 15,
+        PUMAS_EVENT_MEDIUM = 16,
+        PUMAS_EVENT_VERTEX_BREMSSTRAHLUNG = 32,
+        PUMAS_EVENT_VERTEX_PAIR_CREATION = 64,
+        PUMAS_EVENT_VERTEX_PHOTONUCLEAR = 128,
+        PUMAS_EVENT_VERTEX_DELTA_RAY = 256,
+        PUMAS_EVENT_VERTEX_DEL = 480,
+        PUMAS_EVENT_VERTEX_COULOMB = 512,
+        PUMAS_EVENT_VERTEX_DECAY = 1024,
+        PUMAS_EVENT_VERTEX = 2016,
+        PUMAS_EVENT_WEIGHT = 2048,
+        PUMAS_EVENT_START = 4096,
+        PUMAS_EVENT_STOP = 8192
+};
+
+/* Monte Carlo state, include/pumas.h:289-309 (104 bytes). */
+struct pumas_state {
+        double charge;
+        double energy;
+        double distance;
+        double grammage;
+        double time;
+        double weight;
+        double position[3];
+        double direction[3];
+        int decayed;
+};
+
+/* Local medium properties, include/pumas.h:314-321. */
+struct pumas_locals {
+        double density;
+        double magnet[3];
+};
+
+/* Geometry step checking modes, include/pumas.h:461-476. */
+enum pumas_step { PUMAS_STEP_CHECK = 0, PUMAS_STEP_RAW };
+
+/* Context modes and limits, include/pumas.h:554-594. */
+struct pumas_context_mode {
+        enum pumas_mode energy_loss;
+        enum pumas_mode decay;
+        enum pumas_mode direction;
+        enum pumas_mode scattering;
+};
+
+struct pumas_context_limit {
+        double energy;
+        double distance;
+        double grammage;
+        double time;
+};
+#endif /* pumas_h */
+
+/*
+ * The value part of `struct pumas_context` (include/pumas.h:636-672): the
+ * callbacks, recorder and user data of the reference context are host
+ * pointers and have no device counterpart (see the geometry and the Philox
+ * stream below).
+ */
+struct pumas_gpu_context {
+        struct pumas_context_mode mode;
+        enum pumas_event event;
+        struct pumas_context_limit limit;
+        double accuracy;
+};
+
+/* Defaults of pumas_context_create (src/pumas.c:2582-2605) for a muon. */
+void pumas_gpu_context_default(struct pumas_gpu_context * context);
+
+/* ------------------------------------------------------------------------
+ * Physics tables.
+ * ------------------------------------------------------------------------ */
+typedef struct pumas_gpu_physics pumas_gpu_physics;
+
+/*
+ * Upload the physics tables of a PTAB file, once. The file is written by
+ * `pumas_physics_export` from a reference `struct pumas_physics` created with
+ * pumas_physics_create (include/pumas.h:898-906). FP64 tables are uploaded
+ * as is, together with FP32 mirrors for the FP32-bulk precision mode.
+ *
+ * Replaces: the `physics` member of the reference simulation context
+ * (src/pumas.c:426-430), bound by pumas_context_create (include/pumas.h:1214).
+ */
+enum pumas_return pumas_gpu_physics_upload(
+    const char * ptab_path, pumas_gpu_physics ** physics);
+
+/* Same, from a PTAB image already in host memory. */
+enum pumas_return pumas_gpu_physics_upload_memory(
+    const void * ptab, size_t size, pumas_gpu_physics ** physics);
+
+/* Release the device tables; replaces pumas_physics_destroy (pumas.h:924). */
+void pumas_gpu_physics_destroy(pumas_gpu_physics ** physics);
+
+/* Table geometry queries (the PTAB header). */
+int pumas_gpu_physics_n_materials(const pumas_gpu_physics * physics);
+int pumas_gpu_physics_n_energies(const pumas_gpu_physics * physics);
+double pumas_gpu_physics_cutoff(const pumas_gpu_physics * physics);
+
+/* ------------------------------------------------------------------------
+ * Geometry: device media replacing the `pumas_medium_cb` callback
+ * (include/pumas.h:509-511) and the `pumas_locals_cb` callback
+ * (include/pumas.h:348-349). A medium is a material index and a uniform
+ * density; media are identified by index (-1 = outside, i.e. a NULL medium).
+ * ------------------------------------------------------------------------ */
+enum pumas_gpu_geometry_kind {
+        /* One infinite uniform medium: `medium[0]`, no boundary. */
+        PUMAS_GPU_GEOMETRY_INFINITE = 0,
+        /* Horizontal layers: medium i fills z in [z[i], z[i+1]). */
+        PUMAS_GPU_GEOMETRY_SLABS = 1,
+        /* Heightmap terrain: medium 0 is the air above the ground, medium
+         * 1 + (iy * nx + ix) the rock column of cell (ix, iy). */
+        PUMAS_GPU_GEOMETRY_TERRAIN = 2,
+        /* Regular voxel grid: medium 1 + voxel index; material per voxel. */
+        PUMAS_GPU_GEOMETRY_VOXELS = 3
+};
+
+struct pumas_gpu_medium {
+        int material;
+        /* Density in kg/m^3; zero or less selects the material default. */
+        double density;
+};
+
+struct pumas_gpu_geometry_desc {
+        enum pumas_gpu_geometry_kind kind;
+        /* Step checking mode reported by the geometry for proposed steps. */
+        enum pumas_step step_type;
+        /* INFINITE: media[0]. SLABS: media[0..n_layers-1]. TERRAIN: media[0]
+         * is the air, media[1] the ground material (density scaled per
+         * column). VOXELS: media[0..n_materials-1], one per voxel class. */
+        const struct pumas_gpu_medium * media;
+        int n_media;
+        /* SLABS: n_layers + 1 ascending z boundaries, in m. */
+        const double * z;
+        int n_layers;
+        /* TERRAIN / VOXELS grid. */
+        int nx, ny, nz;
+        double x0, y0, z0;       /* grid origin, in m */
+        double dx, dy, dz;       /* cell sizes, in m */
+        const float * height;    /* TERRAIN: [ny][nx] node heights, in m */
+        const float * density;   /* TERRAIN: [ny][nx] column densities;
+                                    VOXELS: [nz][ny][nx] densities, kg/m^3 */
+        const uint8_t * label;   /* VOXELS: [nz][ny][nx] medium class */
+        double z_top;            /* TERRAIN: top of the air layer, in m */
+};
+
+typedef struct pumas_gpu_geometry pumas_gpu_geometry;
+
+enum pumas_return pumas_gpu_geometry_create(const pumas_gpu_physics * physics,
+    const struct pumas_gpu_geometry_desc * desc, pumas_gpu_geometry ** geometry);
+void pumas_gpu_geometry_destroy(pumas_gpu_geometry ** geometry);
+
+/* ------------------------------------------------------------------------
+ * Transport.
+ * ------------------------------------------------------------------------ */
+enum pumas_gpu_precision {
+        /* FP64 everywhere, bit-level parity target with the reference. */
+        PUMAS_GPU_PRECISION_FP64 = 0,
+        /* FP32 table interpolation and sampling, FP64 accumulators. */
+        PUMAS_GPU_PRECISION_FP32_BULK = 1
+};
+
+struct pumas_gpu_options {
+        enum pumas_gpu_precision precision;
+        /* Threads per block (multiple of 64; 0 = default). */
+        int block_size;
+        /* Resident blocks per CU for the persistent kernel (0 = default). */
+        int blocks_per_cu;
+        /* Safety cap on Monte Carlo steps per history (0 = 10^7). */
+        int max_steps;
+};
+
+/* Optional per-history outputs (device pointers, each may be NULL). */
+struct pumas_gpu_outputs {
+        enum pumas_event * events;  /* [n] end condition, as `*event` */
+        int32_t * media;            /* [n][2] start / end medium index */
+        uint32_t * steps;           /* [n] Monte Carlo steps taken */
+        uint32_t * draws;           /* [n] random numbers drawn */
+        int32_t * rc;               /* [n] per history enum pumas_return */
+};
+
+/* Aggregate statistics of one call (host struct). */
+struct pumas_gpu_stats {
+        uint64_t steps;         /* total Monte Carlo steps */
+        uint64_t histories;     /* histories transported */
+        double kernel_ms;       /* transport kernel time (HIP events) */
+};
+
+/*
+ * Transport `n` histories stored in device memory (`states`, AoS
+ * struct pumas_state, updated in place), with the physics, context and
+ * geometry given. History i draws from the Philox stream keyed by `seed` at
+ * counter (first_history + i, draw index), the same stream the reference
+ * receives through `context->random` in the oracle harness.
+ *
+ * Returns the first error raised by any history (catch semantics of
+ * include/pumas.h:1146-1184); per-history codes go to outputs->rc.
+ * `stream` is a hipStream_t (NULL = default stream). The call is
+ * asynchronous unless `stats` is requested.
+ *
+ * Replaces: pumas_context_transport (include/pumas.h:1046-1048) called once
+ * per history, e.g. examples/pumas/straight.c:133-179.
+ */
+enum pumas_return pumas_gpu_transport(const pumas_gpu_physics * physics,
+    const struct pumas_gpu_context * context,
+    const pumas_gpu_geometry * geometry, struct pumas_state * states,
+    size_t n, uint64_t seed, uint64_t first_history,
+    const struct pumas_gpu_outputs * outputs,
+    const struct pumas_gpu_options * options, struct pumas_gpu_stats * stats,
+    void * stream);
+
+/*
+ * Host-memory convenience: copies `states` (and requested outputs) to the
+ * device, transports them and copies the results back. Output pointers in
+ * `outputs` are host pointers here.
+ */
+enum pumas_return pumas_gpu_transport_host(const pumas_gpu_physics * physics,
+    const struct pumas_gpu_context * context,
+    const pumas_gpu_geometry * geometry, struct pumas_state * states,
+    size_t n, uint64_t seed, uint64_t first_history,
+    const struct pumas_gpu_outputs * outputs,
+    const struct pumas_gpu_options * options, struct pumas_gpu_stats * stats);
+
+/* Message of the last error raised on the calling thread, or "". */
+const char * pumas_gpu_error_message(void);
+
+/* Library version string. */
+const char * pumas_gpu_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PUMAS_GPU_H */

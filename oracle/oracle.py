@@ -1,0 +1,171 @@
+"""Python access to the oracle (test infrastructure only).
+
+The oracle is the reference PUMAS library compiled from its own source by
+oracle/Makefile into oracle/_ref/libpumas_ref.so, with the harness of
+oracle/harness.c. Only tests, the smoke check and the CPU baseline of
+bench.py use it, as a checker.
+"""
+import ctypes
+import os
+import shutil
+import tempfile
+
+import numpy as np
+
+from pumas_amd import _abi
+from pumas_amd._abi import STATE_DTYPE
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+LIBRARY = os.path.join(HERE, "_ref", "libpumas_ref.so")
+MDF = {
+    "standard_c005": (os.path.join(REPO, "pumas_amd", "export",
+                                   "materials_standard.xml"), 0.05),
+    "iron_c001": (os.path.join(REPO, "pumas_amd", "export",
+                               "materials_iron.xml"), 0.01),
+}
+
+_lib = None
+
+
+def available():
+    return os.path.exists(LIBRARY)
+
+
+def library():
+    global _lib
+    if _lib is None:
+        if not available():
+            raise FileNotFoundError(
+                f"{LIBRARY} is missing; build it with `make -C oracle`")
+        lib = ctypes.CDLL(LIBRARY)
+        vp, P = ctypes.c_void_p, ctypes.POINTER
+        lib.ref_physics_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p,
+                                           ctypes.c_double, P(vp)]
+        lib.ref_physics_create.restype = ctypes.c_int
+        lib.ref_physics_destroy.argtypes = [vp]
+        lib.ref_physics_destroy.restype = None
+        lib.ref_physics_export.argtypes = [vp, ctypes.c_char_p]
+        lib.ref_physics_export.restype = ctypes.c_int
+        lib.ref_geometry_create.argtypes = [P(_abi.GeometryDesc)]
+        lib.ref_geometry_create.restype = vp
+        lib.ref_geometry_destroy.argtypes = [vp]
+        lib.ref_geometry_destroy.restype = None
+        lib.ref_transport.argtypes = [
+            vp, P(_abi.GpuContext), vp, vp, ctypes.c_int64, ctypes.c_uint64,
+            ctypes.c_uint64, vp, vp, vp, vp, vp, ctypes.c_int,
+            P(ctypes.c_uint64)]
+        lib.ref_transport.restype = ctypes.c_double
+        lib.ref_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint32,
+                                    ctypes.c_uint64, ctypes.c_uint32]
+        lib.ref_uniform.restype = ctypes.c_double
+        lib.ref_philox4x32_10.argtypes = [P(ctypes.c_uint32),
+                                          P(ctypes.c_uint32),
+                                          P(ctypes.c_uint32)]
+        lib.ref_philox4x32_10.restype = None
+        lib.ref_cel.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_double]
+        lib.ref_cel.restype = ctypes.c_double
+        lib.ref_dcs_evaluate.argtypes = [vp, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_double, ctypes.c_double]
+        lib.ref_dcs_evaluate.restype = ctypes.c_double
+        lib.ref_dcs_exact.argtypes = [vp, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_double, ctypes.c_double]
+        lib.ref_dcs_exact.restype = ctypes.c_double
+        lib.ref_dilog.argtypes = [ctypes.c_double]
+        lib.ref_dilog.restype = ctypes.c_double
+        _lib = lib
+    return _lib
+
+
+class RefPhysics:
+    """Reference physics created by pumas_physics_create from the MDF the
+    PTAB file of the same name was exported from."""
+
+    def __init__(self, name):
+        mdf, cutoff = MDF[name]
+        self.name = name
+        self._dedx = tempfile.mkdtemp(prefix="pumas_dedx_")
+        self.handle = ctypes.c_void_p()
+        rc = library().ref_physics_create(mdf.encode(), self._dedx.encode(),
+                                          cutoff, ctypes.byref(self.handle))
+        if rc != 0:
+            raise RuntimeError(f"pumas_physics_create failed ({rc})")
+
+    def export(self, path):
+        rc = library().ref_physics_export(self.handle, str(path).encode())
+        if rc != 0:
+            raise RuntimeError("export failed")
+
+    def cel(self, which, scheme, material, x):
+        return library().ref_cel(self.handle, which, scheme, material, x)
+
+    def dcs_evaluate(self, process, element, K, q):
+        return library().ref_dcs_evaluate(self.handle, process, element, K, q)
+
+    def dcs_exact(self, process, element, K, q):
+        return library().ref_dcs_exact(self.handle, process, element, K, q)
+
+    def close(self):
+        if self.handle:
+            library().ref_physics_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+        shutil.rmtree(self._dedx, ignore_errors=True)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RefGeometry:
+    def __init__(self, description):
+        self.description = description
+        self._desc = description.to_c()
+        self.handle = library().ref_geometry_create(ctypes.byref(self._desc))
+
+    def __del__(self):
+        try:
+            library().ref_geometry_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class RefResult:
+    pass
+
+
+def transport(physics, context, geometry, states, seed=0, first=0,
+              threads=1):
+    """Transport `states` (STATE_DTYPE, copied) with the reference."""
+    states = np.array(states, dtype=STATE_DTYPE, copy=True)
+    n = len(states)
+    r = RefResult()
+    r.states = states
+    r.events = np.zeros(n, np.int32)
+    r.media = np.zeros((n, 2), np.int32)
+    r.steps = np.zeros(n, np.uint32)
+    r.draws = np.zeros(n, np.uint32)
+    r.rc = np.zeros(n, np.int32)
+    total = ctypes.c_uint64()
+    ctx = context.to_c()
+    r.seconds = library().ref_transport(
+        physics.handle, ctypes.byref(ctx), geometry.handle,
+        states.ctypes.data, n, seed, first, r.events.ctypes.data,
+        r.media.ctypes.data, r.steps.ctypes.data, r.draws.ctypes.data,
+        r.rc.ctypes.data, threads, ctypes.byref(total))
+    r.total_steps = int(total.value)
+    return r
+
+
+def uniform(seed, stream, history, index):
+    return library().ref_uniform(seed, stream, history, index)
+
+
+def philox(ctr, key):
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    library().ref_philox4x32_10(c, k, o)
+    return tuple(o)

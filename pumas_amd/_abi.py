@@ -1,0 +1,207 @@
+"""ctypes and numpy mirrors of the C ABI in include/pumas_gpu.h.
+
+The state and context layouts are those of the reference public header
+(include/pumas.h:289-309 for `struct pumas_state`, include/pumas.h:554-594
+for the context mode and limits), so that arrays can be shared with the
+reference library unchanged.
+"""
+import ctypes
+import enum
+
+import numpy as np
+
+
+class Mode(enum.IntEnum):
+    """enum pumas_mode, include/pumas.h:90-140 (aliased values kept)."""
+    DISABLED = -1
+    CSDA = 0
+    MIXED = 1
+    STRAGGLED = 2
+    WEIGHTED = 0
+    RANDOMISED = 1
+    FORWARD = 0
+    BACKWARD = 1
+
+
+class Event(enum.IntFlag):
+    """enum pumas_event, include/pumas.h:199-236."""
+    NONE = 0
+    LIMIT_ENERGY = 1
+    LIMIT_DISTANCE = 2
+    LIMIT_GRAMMAGE = 4
+    LIMIT_TIME = 8
+    LIMIT = 15
+    MEDIUM = 16
+    VERTEX_BREMSSTRAHLUNG = 32
+    VERTEX_PAIR_CREATION = 64
+    VERTEX_PHOTONUCLEAR = 128
+    VERTEX_DELTA_RAY = 256
+    VERTEX_DEL = 480
+    VERTEX_COULOMB = 512
+    VERTEX_DECAY = 1024
+    VERTEX = 2016
+    WEIGHT = 2048
+    START = 4096
+    STOP = 8192
+
+
+class Return(enum.IntEnum):
+    """enum pumas_return, include/pumas.h:143-196."""
+    SUCCESS = 0
+    ACCURACY_ERROR = 1
+    END_OF_FILE = 2
+    DECAY_ERROR = 3
+    DENSITY_ERROR = 4
+    DIRECTION_ERROR = 5
+    INCOMPLETE_FILE = 6
+    INDEX_ERROR = 7
+    PHYSICS_ERROR = 8
+    INTERNAL_ERROR = 9
+    IO_ERROR = 10
+    FORMAT_ERROR = 11
+    MEDIUM_ERROR = 12
+    MEMORY_ERROR = 13
+    MODEL_ERROR = 14
+    MISSING_LIMIT = 15
+    MISSING_RANDOM = 16
+    PATH_ERROR = 17
+    RAISE_ERROR = 18
+    TOO_LONG = 19
+    UNDEFINED_MDF = 20
+    UNKNOWN_ELEMENT = 21
+    UNKNOWN_MATERIAL = 22
+    UNKNOWN_PARTICLE = 23
+    VALUE_ERROR = 24
+
+
+class GeometryKind(enum.IntEnum):
+    INFINITE = 0
+    SLABS = 1
+    TERRAIN = 2
+    VOXELS = 3
+
+
+class Step(enum.IntEnum):
+    """enum pumas_step, include/pumas.h:461-476."""
+    CHECK = 0
+    RAW = 1
+
+
+class Precision(enum.IntEnum):
+    FP64 = 0
+    FP32_BULK = 1
+
+
+# struct pumas_state as a numpy record (104 bytes, 8-byte aligned).
+STATE_DTYPE = np.dtype([
+    ("charge", "f8"), ("energy", "f8"), ("distance", "f8"),
+    ("grammage", "f8"), ("time", "f8"), ("weight", "f8"),
+    ("position", "f8", (3,)), ("direction", "f8", (3,)),
+    ("decayed", "i4"), ("_pad", "i4"),
+])
+assert STATE_DTYPE.itemsize == 104
+
+
+class ContextMode(ctypes.Structure):
+    _fields_ = [("energy_loss", ctypes.c_int), ("decay", ctypes.c_int),
+                ("direction", ctypes.c_int), ("scattering", ctypes.c_int)]
+
+
+class ContextLimit(ctypes.Structure):
+    _fields_ = [("energy", ctypes.c_double), ("distance", ctypes.c_double),
+                ("grammage", ctypes.c_double), ("time", ctypes.c_double)]
+
+
+class GpuContext(ctypes.Structure):
+    _fields_ = [("mode", ContextMode), ("event", ctypes.c_int),
+                ("limit", ContextLimit), ("accuracy", ctypes.c_double)]
+
+
+class GpuMedium(ctypes.Structure):
+    _fields_ = [("material", ctypes.c_int), ("density", ctypes.c_double)]
+
+
+class GeometryDesc(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int), ("step_type", ctypes.c_int),
+        ("media", ctypes.POINTER(GpuMedium)), ("n_media", ctypes.c_int),
+        ("z", ctypes.POINTER(ctypes.c_double)), ("n_layers", ctypes.c_int),
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("x0", ctypes.c_double), ("y0", ctypes.c_double),
+        ("z0", ctypes.c_double), ("dx", ctypes.c_double),
+        ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+        ("height", ctypes.POINTER(ctypes.c_float)),
+        ("density", ctypes.POINTER(ctypes.c_float)),
+        ("label", ctypes.POINTER(ctypes.c_uint8)),
+        ("z_top", ctypes.c_double),
+    ]
+
+
+class GpuOutputs(ctypes.Structure):
+    _fields_ = [("events", ctypes.c_void_p), ("media", ctypes.c_void_p),
+                ("steps", ctypes.c_void_p), ("draws", ctypes.c_void_p),
+                ("rc", ctypes.c_void_p)]
+
+
+class GpuOptions(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("block_size", ctypes.c_int),
+                ("blocks_per_cu", ctypes.c_int), ("max_steps", ctypes.c_int)]
+
+
+class GpuStats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_uint64), ("histories", ctypes.c_uint64),
+                ("kernel_ms", ctypes.c_double)]
+
+
+# Every entry point declared by include/pumas_gpu.h.
+EXPORTED_SYMBOLS = (
+    "pumas_gpu_context_default",
+    "pumas_gpu_physics_upload",
+    "pumas_gpu_physics_upload_memory",
+    "pumas_gpu_physics_destroy",
+    "pumas_gpu_physics_n_materials",
+    "pumas_gpu_physics_n_energies",
+    "pumas_gpu_physics_cutoff",
+    "pumas_gpu_geometry_create",
+    "pumas_gpu_geometry_destroy",
+    "pumas_gpu_transport",
+    "pumas_gpu_transport_host",
+    "pumas_gpu_error_message",
+    "pumas_gpu_version",
+)
+
+
+def declare(lib):
+    """Attach argument and result types to the library functions."""
+    P = ctypes.POINTER
+    vp = ctypes.c_void_p
+    lib.pumas_gpu_context_default.argtypes = [P(GpuContext)]
+    lib.pumas_gpu_context_default.restype = None
+    lib.pumas_gpu_physics_upload.argtypes = [ctypes.c_char_p, P(vp)]
+    lib.pumas_gpu_physics_upload.restype = ctypes.c_int
+    lib.pumas_gpu_physics_upload_memory.argtypes = [vp, ctypes.c_size_t, P(vp)]
+    lib.pumas_gpu_physics_upload_memory.restype = ctypes.c_int
+    lib.pumas_gpu_physics_destroy.argtypes = [P(vp)]
+    lib.pumas_gpu_physics_destroy.restype = None
+    lib.pumas_gpu_physics_n_materials.argtypes = [vp]
+    lib.pumas_gpu_physics_n_materials.restype = ctypes.c_int
+    lib.pumas_gpu_physics_n_energies.argtypes = [vp]
+    lib.pumas_gpu_physics_n_energies.restype = ctypes.c_int
+    lib.pumas_gpu_physics_cutoff.argtypes = [vp]
+    lib.pumas_gpu_physics_cutoff.restype = ctypes.c_double
+    lib.pumas_gpu_geometry_create.argtypes = [vp, P(GeometryDesc), P(vp)]
+    lib.pumas_gpu_geometry_create.restype = ctypes.c_int
+    lib.pumas_gpu_geometry_destroy.argtypes = [P(vp)]
+    lib.pumas_gpu_geometry_destroy.restype = None
+    lib.pumas_gpu_transport.argtypes = [
+        vp, P(GpuContext), vp, vp, ctypes.c_size_t, ctypes.c_uint64,
+        ctypes.c_uint64, P(GpuOutputs), P(GpuOptions), P(GpuStats), vp]
+    lib.pumas_gpu_transport.restype = ctypes.c_int
+    lib.pumas_gpu_transport_host.argtypes = [
+        vp, P(GpuContext), vp, vp, ctypes.c_size_t, ctypes.c_uint64,
+        ctypes.c_uint64, P(GpuOutputs), P(GpuOptions), P(GpuStats)]
+    lib.pumas_gpu_transport_host.restype = ctypes.c_int
+    lib.pumas_gpu_error_message.argtypes = []
+    lib.pumas_gpu_error_message.restype = ctypes.c_char_p
+    lib.pumas_gpu_version.argtypes = []
+    lib.pumas_gpu_version.restype = ctypes.c_char_p

@@ -1,0 +1,526 @@
+/*
+ * Device transport engine: common definitions, random stream and the physics
+ * table view with its interpolation routines.
+ *
+ * The engine is written once for the device (hipcc, gfx950). Every routine
+ * that reproduces a reference routine cites it (src/pumas.c:line) and keeps
+ * its floating point operation order, because per-history parity with the
+ * reference is the acceptance test. Host pointers, callbacks and the
+ * per-context index cache of the reference have no counterpart here:
+ * table_index returns the unique bracketing row, which is what the cached
+ * and uncached reference paths both return for the non decreasing tables
+ * checked at export time.
+ */
+#ifndef PUMAS_ENGINE_COMMON_H
+#define PUMAS_ENGINE_COMMON_H
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define PG_FN __device__ __forceinline__
+#define PG_NOINLINE __device__ __noinline__
+#else
+#include <math.h>
+#define PG_FN inline
+#define PG_NOINLINE __attribute__((noinline))
+#endif
+
+namespace pg {
+
+/* Physical constants and tuning parameters, src/pumas.c:53-200. */
+constexpr double ALPHA_EM = 7.2973525693E-03;
+constexpr double HBAR_C = 1.973269804E-16;
+constexpr double BOHR_RADIUS = 0.529177210903E-10;
+constexpr double ELECTRON_MASS = 0.510998910E-03;
+constexpr double ELECTRON_RADIUS = 2.817940285E-15;
+constexpr double PROTON_MASS = 0.938272;
+constexpr double NEUTRON_MASS = 0.939565;
+constexpr double PION_MASS = 0.13957018;
+constexpr double AVOGADRO_NUMBER = 6.02214076E+23;
+constexpr double LARMOR_FACTOR = 0.299792458;
+constexpr double BMC_ALPHA = 2.0;
+constexpr double EPSILON_X = 3E-03;
+constexpr double STEP_MIN = 1E-07;
+constexpr double PI = 3.14159265358979323846;
+constexpr double FLT_EPS = 1.19209290e-07; /* FLT_EPSILON as a double */
+constexpr double DBL_EPS = 2.2204460492503131e-16;
+constexpr double DBL_MAXV = 1.7976931348623157e+308;
+constexpr int N_DEL_PROCESSES = 4;
+
+/* Enum values of include/pumas.h. */
+enum {
+        MODE_DISABLED = -1,
+        MODE_CSDA = 0,
+        MODE_MIXED = 1,
+        MODE_STRAGGLED = 2,
+        MODE_WEIGHTED = 0,
+        MODE_RANDOMISED = 1,
+        MODE_FORWARD = 0,
+        MODE_BACKWARD = 1
+};
+
+enum {
+        EVENT_NONE = 0,
+        EVENT_LIMIT_ENERGY = 1,
+        EVENT_LIMIT_DISTANCE = 2,
+        EVENT_LIMIT_GRAMMAGE = 4,
+        EVENT_LIMIT_TIME = 8,
+        EVENT_LIMIT = 15,
+        EVENT_MEDIUM = 16,
+        EVENT_VERTEX_BREMSSTRAHLUNG = 32,
+        EVENT_VERTEX_PAIR_CREATION = 64,
+        EVENT_VERTEX_PHOTONUCLEAR = 128,
+        EVENT_VERTEX_DELTA_RAY = 256,
+        EVENT_VERTEX_DEL = 480,
+        EVENT_VERTEX_COULOMB = 512,
+        EVENT_VERTEX_DECAY = 1024,
+        EVENT_VERTEX = 2016,
+        EVENT_WEIGHT = 2048
+};
+
+enum {
+        RETURN_SUCCESS = 0,
+        RETURN_ACCURACY_ERROR = 1,
+        RETURN_DECAY_ERROR = 3,
+        RETURN_DENSITY_ERROR = 4,
+        RETURN_DIRECTION_ERROR = 5,
+        RETURN_MEDIUM_ERROR = 12,
+        RETURN_MISSING_LIMIT = 15,
+        RETURN_VALUE_ERROR = 24
+};
+
+enum { STEP_CHECK = 0, STEP_RAW = 1 };
+
+/* ------------------------------------------------------------------------
+ * Counter based random stream: Philox4x32-10 (Salmon et al., SC'11).
+ *
+ * key     = (seed low 32 bits, seed high 32 bits)
+ * counter = (draw index / 2, stream, history id low, history id high)
+ * draw 2k uses output words (x0, x1), draw 2k+1 uses (x2, x3), as the 64-bit
+ * word (x1 << 32 | x0); the uniform is ((w >> 11) + 0.5) 2^-53 in (0, 1).
+ * ------------------------------------------------------------------------ */
+struct Philox4 {
+        uint32_t x[4];
+};
+
+PG_FN Philox4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2,
+    uint32_t c3, uint32_t k0, uint32_t k1)
+{
+        const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+        const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+        for (int r = 0; r < 10; r++) {
+                const uint64_t p0 = (uint64_t)M0 * c0;
+                const uint64_t p1 = (uint64_t)M1 * c2;
+                const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+                const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+                c0 = hi1 ^ c1 ^ k0;
+                c1 = lo1;
+                c2 = hi0 ^ c3 ^ k1;
+                c3 = lo0;
+                k0 += W0;
+                k1 += W1;
+        }
+        Philox4 out;
+        out.x[0] = c0;
+        out.x[1] = c1;
+        out.x[2] = c2;
+        out.x[3] = c3;
+        return out;
+}
+
+/* Stream identifiers. */
+enum { STREAM_SAMPLING = 0, STREAM_TRANSPORT = 1 };
+
+struct Rng {
+        uint32_t k0, k1;
+        uint32_t h0, h1;
+        uint32_t stream;
+        uint32_t count;
+
+        PG_FN void init(uint64_t seed, uint64_t history, uint32_t stream_,
+            uint32_t count_)
+        {
+                k0 = (uint32_t)seed;
+                k1 = (uint32_t)(seed >> 32);
+                h0 = (uint32_t)history;
+                h1 = (uint32_t)(history >> 32);
+                stream = stream_;
+                count = count_;
+        }
+
+        PG_FN double uniform()
+        {
+                const uint32_t i = count++;
+                const Philox4 r =
+                    philox4x32_10(i >> 1, stream, h0, h1, k0, k1);
+                uint64_t w;
+                if (i & 1u)
+                        w = ((uint64_t)r.x[3] << 32) | r.x[2];
+                else
+                        w = ((uint64_t)r.x[1] << 32) | r.x[0];
+                return ((double)(w >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        }
+};
+
+/* ------------------------------------------------------------------------
+ * Single precision logarithm and exponential used by the FP32 DCS spline.
+ * The reference calls the C library logf / expf (src/pumas.c:11298-11346);
+ * here the value is computed in double precision and rounded once, i.e. a
+ * correctly rounded result except for rare double rounding cases.
+ * ------------------------------------------------------------------------ */
+PG_FN float logf_cr(float x) { return (float)log((double)x); }
+PG_FN float expf_cr(float x) { return (float)exp((double)x); }
+
+/* ------------------------------------------------------------------------
+ * Physics tables in device memory (PTAB sections, see ptab.h).
+ * ------------------------------------------------------------------------ */
+struct PhysicsView {
+        int n_energies, n_materials, n_elements, n_components, n_table_dcs;
+        int max_components;
+        double mass, ctau, cutoff;
+
+        const double *K, *K_dX, *K_dNI_in, *K_dNI_el;
+        const double *X, *X_dK, *X_dT, *T, *T_dK, *dE, *dE_dK;
+        const double *Omega, *Omega_dK, *NI_el, *NI_el_dK, *NI_in, *NI_in_dK;
+        const double *CS, *CS_dK, *CSf, *CSf_dK, *CSn, *CSn_dK;
+        const double *Kt, *a_max, *b_max, *Mu0, *Mu0_dK, *Lb, *Lb_dK;
+        const double *Ms1, *Ms1_dK;
+        const double *material_density, *component_fraction;
+        const double *element_Z, *element_A, *element_I;
+        const double *element_nscr, *element_amplitude, *element_exponent;
+        const double *element_rn, *element_lrad;
+        const float *DCS, *DCS_x, *DCS_envelope;
+        const int *elements_in, *component_element, *element_index;
+        const int *material_component0; /* first component of a material */
+};
+
+/* Transport configuration (value part of struct pumas_context). */
+struct Config {
+        int energy_loss, decay, direction, scattering;
+        int event;
+        double limit_energy, limit_distance, limit_grammage, limit_time;
+        double accuracy;
+};
+
+/* Table accessors, src/pumas.c:4260-4752. */
+PG_FN int scheme2(int scheme) { return (scheme > MODE_MIXED) ? MODE_MIXED : scheme; }
+
+PG_FN const double * tab2(const PhysicsView & p, const double * t, int scheme,
+    int material, int row)
+{
+        return t + (scheme2(scheme) * p.n_materials + material) * p.n_energies +
+            row;
+}
+
+PG_FN const double * tab1(
+    const PhysicsView & p, const double * t, int material, int row)
+{
+        return t + material * p.n_energies + row;
+}
+
+PG_FN const double * tab_NI_el(const PhysicsView & p, const double * t,
+    int scheme, int material, int row)
+{
+        const int s = (scheme >= MODE_MIXED) ? MODE_MIXED : MODE_CSDA;
+        return t + (s * p.n_materials + material) * p.n_energies + row;
+}
+
+PG_FN const double * tab_Ms1(const PhysicsView & p, const double * t,
+    int scheme, int material, int row)
+{
+        return t + ((scheme2(scheme) + 1) * p.n_materials + material) *
+            p.n_energies + row;
+}
+
+PG_FN double b_max(const PhysicsView & p, int scheme, int material)
+{
+        return p.b_max[scheme2(scheme) * p.n_materials + material];
+}
+
+/* Hermite cubic, src/pumas.c:4118-4125. */
+PG_FN double pchip(double t, double p0, double p1, double m0, double m1)
+{
+        const double c2 = -3 * (p0 - p1) - 2 * m0 - m1;
+        const double c3 = 2 * (p0 - p1) + m0 + m1;
+        return p0 + t * (m0 + t * (c2 + t * c3));
+}
+
+/*
+ * Row i1 with table[i1] <= value < table[i1 + 1], -1 below the table and
+ * n - 1 above it: the result of table_index / table_bracket
+ * (src/pumas.c:4174-4233).
+ */
+PG_FN int table_index(const double * table, int n, double value)
+{
+        const int imax = n - 1;
+        if (value < table[0]) return -1;
+        if (value >= table[imax]) return imax;
+        int i1 = 0, i2 = imax;
+        do {
+                const int i3 = (i1 + i2) / 2;
+                if (value >= table[i3])
+                        i1 = i3;
+                else
+                        i2 = i3;
+        } while (i2 - i1 >= 2);
+        return i1;
+}
+
+/* src/pumas.c:4146-4157. */
+PG_FN double interpolate(const PhysicsView & p, const double * tx,
+    const double * ty, const double * tm, double x)
+{
+        const int i1 = table_index(tx, p.n_energies, x);
+        const int i2 = i1 + 1;
+        const double dX = tx[i2] - tx[i1];
+        const double t = (x - tx[i1]) / dX;
+        const double m1 = tm[i1] * dX;
+        const double m2 = (i2 > 1) ? tm[i2] * dX : m1;
+        return pchip(t, ty[i1], ty[i2], m1, m2);
+}
+
+/* cel_grammage, src/pumas.c:3644-3669. */
+PG_FN double cel_grammage(
+    const PhysicsView & p, int scheme, int material, double kinetic)
+{
+        const int imax = p.n_energies - 1;
+        if (kinetic < p.K[0]) return 0.;
+        if (kinetic >= p.K[imax]) {
+                const double a = p.a_max[material];
+                const double b = b_max(p, scheme, material);
+                const double K0 = p.K[imax];
+                const double K1 = a / b + p.mass;
+                return *tab2(p, p.X, scheme, material, imax) +
+                    1. / b * log((kinetic + K1) / (K0 + K1));
+        }
+        return interpolate(p, p.K, tab2(p, p.X, scheme, material, 0),
+            tab2(p, p.X_dK, scheme, material, 0), kinetic);
+}
+
+/* cel_grammage_as_time, src/pumas.c:3680-3709. */
+PG_FN double cel_grammage_as_time(
+    const PhysicsView & p, int scheme, int material, double time)
+{
+        const int imax = p.n_energies - 1;
+        const double * T = tab2(p, p.T, scheme, material, 0);
+        if (time < T[0]) return 0.;
+        if (time >= T[imax]) {
+                const double a = p.a_max[material];
+                const double b = b_max(p, scheme, material);
+                const double E0 = p.K[imax] + p.mass;
+                const double tmax = T[imax];
+                const double r =
+                    E0 / (a + b * E0) * exp(a * (time - tmax) / p.mass);
+                const double kinetic = a * r / (1. - b * r) - p.mass;
+                const double K1 = a / b + p.mass;
+                return *tab2(p, p.X, scheme, material, imax) +
+                    1. / b * log((kinetic + K1) / (E0 - p.mass + K1));
+        }
+        return interpolate(p, T, tab2(p, p.X, scheme, material, 0),
+            tab2(p, p.X_dT, scheme, material, 0), time);
+}
+
+/* cel_proper_time, src/pumas.c:3720-3745. */
+PG_FN double cel_proper_time(
+    const PhysicsView & p, int scheme, int material, double kinetic)
+{
+        const int imax = p.n_energies - 1;
+        if (kinetic < p.K[0]) return 0.;
+        if (kinetic >= p.K[imax]) {
+                const double a = p.a_max[material];
+                const double b = b_max(p, scheme, material);
+                const double E0 = p.K[imax] + p.mass;
+                const double E1 = kinetic + p.mass;
+                return *tab2(p, p.T, scheme, material, imax) +
+                    p.mass / a * log((E1 / E0) * (a + b * E0) / (a + b * E1));
+        }
+        return interpolate(p, p.K, tab2(p, p.T, scheme, material, 0),
+            tab2(p, p.T_dK, scheme, material, 0), kinetic);
+}
+
+/* cel_kinetic_energy, src/pumas.c:3758-3784. */
+PG_FN double cel_kinetic_energy(
+    const PhysicsView & p, int scheme, int material, double grammage)
+{
+        const int imax = p.n_energies - 1;
+        const double * X = tab2(p, p.X, scheme, material, 0);
+        if (grammage < X[0]) return 0.;
+        if (grammage >= X[imax]) {
+                const double a = p.a_max[material];
+                const double b = b_max(p, scheme, material);
+                const double K0 = p.K[imax];
+                const double K1 = a / b + p.mass;
+                return (K0 + K1) * exp(b * (grammage - X[imax])) - K1;
+        }
+        return interpolate(
+            p, X, p.K, tab2(p, p.K_dX, scheme, material, 0), grammage);
+}
+
+/* cel_energy_loss, src/pumas.c:3796-3814. */
+PG_FN double cel_energy_loss(
+    const PhysicsView & p, int scheme, int material, double kinetic)
+{
+        const int imax = p.n_energies - 1;
+        if (kinetic < p.K[0]) return 0.;
+        if (kinetic >= p.K[imax]) {
+                return p.a_max[material] +
+                    b_max(p, scheme, material) * (kinetic + p.mass);
+        }
+        return interpolate(p, p.K, tab2(p, p.dE, scheme, material, 0),
+            tab2(p, p.dE_dK, scheme, material, 0), kinetic);
+}
+
+/* cel_straggling, src/pumas.c:3826-3846. */
+PG_FN double cel_straggling(const PhysicsView & p, int material, double kinetic)
+{
+        const int imax = p.n_energies - 1;
+        if (kinetic < p.K[0]) return 0.;
+        const double Kmax = p.K[imax];
+        if (kinetic >= Kmax) {
+                const double r = (kinetic + p.mass) / (Kmax + p.mass);
+                return *tab1(p, p.Omega, material, imax) * r * r;
+        }
+        return interpolate(p, p.K, tab1(p, p.Omega, material, 0),
+            tab1(p, p.Omega_dK, material, 0), kinetic);
+}
+
+/* del_cross_section, src/pumas.c:3895-3912. */
+PG_FN double del_cross_section(
+    const PhysicsView & p, int material, double kinetic)
+{
+        const int imax = p.n_energies - 1;
+        if (kinetic < p.K[0]) return 0.;
+        if (kinetic >= p.K[imax]) return *tab1(p, p.CS, material, imax);
+        return interpolate(p, p.K, tab1(p, p.CS, material, 0),
+            tab1(p, p.CS_dK, material, 0), kinetic);
+}
+
+/* del_interaction_length, src/pumas.c:3923-3947. */
+PG_FN double del_interaction_length(
+    const PhysicsView & p, int material, double kinetic)
+{
+        const int imax = p.n_energies - 1;
+        if (kinetic < p.K[0]) return 0.;
+        if (kinetic >= p.K[imax]) {
+                const double k0 = p.K[imax];
+                const double a0 = p.a_max[material];
+                const double b0 = b_max(p, MODE_MIXED, material);
+                const double cs = *tab1(p, p.CS, material, imax);
+                const double dZ = cs / b0 *
+                    log((a0 + b0 * (kinetic + p.mass)) /
+                        (a0 + b0 * (k0 + p.mass)));
+                return *tab1(p, p.NI_in, material, imax) + dZ;
+        }
+        return interpolate(p, p.K, tab1(p, p.NI_in, material, 0),
+            tab1(p, p.NI_in_dK, material, 0), kinetic);
+}
+
+/* del_kinetic_from_interaction_length, src/pumas.c:3959-3987. */
+PG_FN double del_kinetic_from_interaction_length(
+    const PhysicsView & p, int material, double nI)
+{
+        const int imax = p.n_energies - 1;
+        const double * NI = tab1(p, p.NI_in, material, 0);
+        if (nI < NI[0]) return 0.;
+        if (nI >= NI[imax]) {
+                const double k0 = p.K[imax];
+                const double a0 = p.a_max[material];
+                const double b0 = b_max(p, MODE_MIXED, material);
+                const double cs = *tab1(p, p.CS, material, imax);
+                const double nI0 = NI[imax];
+                return ((a0 + b0 * (k0 + p.mass)) * exp(b0 * (nI - nI0) / cs) -
+                           a0) / b0 - p.mass;
+        }
+        return interpolate(
+            p, NI, p.K, tab1(p, p.K_dNI_in, material, 0), nI);
+}
+
+/* ehs_interaction_length, src/pumas.c:3998-4023. */
+PG_FN double ehs_interaction_length(
+    const PhysicsView & p, int scheme, int material, double kinetic)
+{
+        const int imax = p.n_energies - 1;
+        const double * NI = tab_NI_el(p, p.NI_el, scheme, material, 0);
+        if (kinetic < p.K[0]) return 0.;
+        if (kinetic >= p.K[imax]) {
+                const double k0 = p.K[imax - 1];
+                const double k1 = p.K[imax];
+                const double nI0 = NI[imax - 1];
+                const double nI1 = NI[imax];
+                return nI1 + (kinetic - k1) * (nI1 - nI0) / (k1 - k0);
+        }
+        return interpolate(p, p.K, NI,
+            tab_NI_el(p, p.NI_el_dK, scheme, material, 0), kinetic);
+}
+
+/* ehs_kinetic_from_interaction_length, src/pumas.c:4034-4058. */
+PG_FN double ehs_kinetic_from_interaction_length(
+    const PhysicsView & p, int scheme, int material, double nI)
+{
+        const int imax = p.n_energies - 1;
+        const double * NI = tab_NI_el(p, p.NI_el, scheme, material, 0);
+        if (nI < NI[0]) return 0.;
+        if (nI >= NI[imax]) {
+                const double k0 = p.K[imax - 1];
+                const double k1 = p.K[imax];
+                const double nI0 = NI[imax - 1];
+                const double nI1 = NI[imax];
+                return k1 + (nI - nI1) * (k1 - k0) / (nI1 - nI0);
+        }
+        /* K_dNI_el shares the two-scheme layout of the other CEL tables. */
+        return interpolate(
+            p, NI, p.K, tab2(p, p.K_dNI_el, scheme, material, 0), nI);
+}
+
+/* table_get_msc, src/pumas.c:4077-4111. */
+PG_FN void table_get_msc(const PhysicsView & p, int scheme, int material,
+    double kinetic, double * mu0, double * invlb1)
+{
+        const int imax = p.n_energies - 1;
+        if (kinetic < p.K[1]) {
+                *mu0 = *tab1(p, p.Mu0, material, 1);
+                *invlb1 = *tab_Ms1(p, p.Ms1, scheme, material, 1) *
+                    sqrt(p.K[1] / kinetic);
+        } else if (kinetic >= p.K[imax]) {
+                *mu0 = *tab1(p, p.Mu0, material, imax);
+                *invlb1 = *tab_Ms1(p, p.Ms1, scheme, material, imax) *
+                    p.K[imax] / kinetic;
+        } else {
+                const double * x = p.K;
+                const int i1 = table_index(x, p.n_energies, kinetic);
+                x += i1;
+                const double * y0 = tab1(p, p.Mu0, material, i1);
+                const double * m0 = tab1(p, p.Mu0_dK, material, i1);
+                const double * y1 = tab_Ms1(p, p.Ms1, scheme, material, i1);
+                const double * m1 = tab_Ms1(p, p.Ms1_dK, scheme, material, i1);
+                const double dx = x[1] - x[0];
+                const double t = (kinetic - x[0]) / dx;
+                *mu0 = pchip(t, y0[0], y0[1], m0[0] * dx, m0[1] * dx);
+                *invlb1 = pchip(t, y1[0], y1[1], m1[0] * dx, m1[1] * dx);
+        }
+}
+
+/* coulomb_ehs_length, src/pumas.c:7446-7466. */
+PG_FN double coulomb_ehs_length(
+    const PhysicsView & p, int material, double kinetic)
+{
+        const double p2 = kinetic * (kinetic + 2. * p.mass);
+        const int imax = p.n_energies - 1;
+        const double * Lb = tab1(p, p.Lb, material, 0);
+        if (kinetic < p.K[1]) {
+                return Lb[1] / p2;
+        } else if (kinetic >= p.K[imax]) {
+                return Lb[imax] / p2;
+        } else {
+                const int i1 = table_index(p.K, p.n_energies, kinetic);
+                const int i2 = i1 + 1;
+                const double h = (kinetic - p.K[i1]) / (p.K[i2] - p.K[i1]);
+                return (Lb[i1] + h * (Lb[i2] - Lb[i1])) / p2;
+        }
+}
+
+} /* namespace pg */
+
+#endif

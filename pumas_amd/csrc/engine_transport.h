@@ -1,0 +1,1561 @@
+/*
+ * Device transport engine: one Monte Carlo history, from a start state to an
+ * end condition, as pumas_context_transport (src/pumas.c:3114-3251) with the
+ * geometry and random stream of this library.
+ *
+ * A history keeps the reference's per-context stepping memory
+ * (struct simulation_context, src/pumas.c:426-477) in the Track below, so
+ * that draws, step lengths and events follow the reference one for one.
+ */
+#ifndef PUMAS_ENGINE_TRANSPORT_H
+#define PUMAS_ENGINE_TRANSPORT_H
+
+#include "engine_interactions.h"
+#include "geometry.h"
+
+namespace pg {
+
+constexpr int MAX_COMPONENTS = 8;
+
+struct Track {
+        /* struct pumas_state */
+        double charge, energy, distance, grammage, time, weight;
+        double pos[3], dir[3];
+        int decayed;
+        /* Stepping memory of the reference simulation context. */
+        int step_first, step_event, step_foreseen;
+        double step_X_limit, step_invlb1, lifetime;
+        int randn_done;
+        double randn_next;
+        /* Random stream and step counter. */
+        Rng rng;
+        uint32_t steps;
+        int max_steps;
+};
+
+struct Engine {
+        const PhysicsView & p;
+        const ModelView & mv;
+        const Config & c;
+        const pg_geometry & g;
+
+        PG_FN double sgn() const
+        {
+                return (c.direction == MODE_FORWARD) ? 1. : -1.;
+        }
+
+        /* Medium callback with a step proposal (counts Monte Carlo steps). */
+        PG_FN int medium_step(Track & t, int * medium, double * step) const
+        {
+                t.steps++;
+                return geo_locate(&g, t.pos, t.dir, sgn(), medium, step);
+        }
+
+        /* Location only medium callback. */
+        PG_FN int medium_at(const double * pos, const Track & t) const
+        {
+                int m;
+                geo_locate(&g, pos, t.dir, sgn(), &m, 0);
+                return m;
+        }
+
+        /* transport_set_locals, src/pumas.c:5528-5552: every device medium
+         * is uniform, hence a null locals step. */
+        PG_FN double locals_density(int medium, int * material) const
+        {
+                double density;
+                geo_medium_properties(&g, medium, material, &density);
+                if (density <= 0.) density = p.material_density[*material];
+                return density;
+        }
+
+        /* step_randn, src/pumas.c:7016-7029. */
+        PG_FN double randn(Track & t) const
+        {
+                t.randn_done = !t.randn_done;
+                if (!t.randn_done) return t.randn_next;
+                const double r = sqrt(-2. * log(t.rng.uniform()));
+                const double phi = 2. * PI * t.rng.uniform();
+                const double cp = cos(phi);
+                const double s = sin(phi);
+                t.randn_next = r * cp;
+                return r * s;
+        }
+
+        /* step_rotate_direction, src/pumas.c:7042-7095. */
+        PG_FN void rotate(Track & t, double mu) const
+        {
+                double * const direction = t.dir;
+                const double cos_theta = 1 - 2 * mu;
+                const double stsq = 4 * mu * (1. - mu);
+                if (stsq <= 0.) return;
+                const double st = sqrt(stsq);
+
+                double u0x = 0., u0y = 0., u0z = 0.;
+                const double a0 = fabs(direction[0]);
+                const double a1 = fabs(direction[1]);
+                const double a2 = fabs(direction[2]);
+                if (a0 > a1) {
+                        if (a0 > a2) {
+                                const double nrm =
+                                    1. / sqrt(direction[0] * direction[0] +
+                                             direction[2] * direction[2]);
+                                u0x = -direction[2] * nrm,
+                                u0z = direction[0] * nrm;
+                        } else {
+                                const double nrm =
+                                    1. / sqrt(direction[1] * direction[1] +
+                                             direction[2] * direction[2]);
+                                u0y = direction[2] * nrm,
+                                u0z = -direction[1] * nrm;
+                        }
+                } else {
+                        if (a1 > a2) {
+                                const double nrm =
+                                    1. / sqrt(direction[0] * direction[0] +
+                                             direction[1] * direction[1]);
+                                u0x = direction[1] * nrm,
+                                u0y = -direction[0] * nrm;
+                        } else {
+                                const double nrm =
+                                    1. / sqrt(direction[1] * direction[1] +
+                                             direction[2] * direction[2]);
+                                u0y = direction[2] * nrm,
+                                u0z = -direction[1] * nrm;
+                        }
+                }
+                const double u1x = u0y * direction[2] - u0z * direction[1];
+                const double u1y = u0z * direction[0] - u0x * direction[2];
+                const double u1z = u0x * direction[1] - u0y * direction[0];
+
+                const double phi = PI * (1. - 2. * t.rng.uniform());
+                const double cp = cos(phi);
+                const double sp = sin(phi);
+                direction[0] =
+                    cos_theta * direction[0] + st * (cp * u0x + sp * u1x);
+                direction[1] =
+                    cos_theta * direction[1] + st * (cp * u0y + sp * u1y);
+                direction[2] =
+                    cos_theta * direction[2] + st * (cp * u0z + sp * u1z);
+        }
+
+        /* transport_with_csda, src/pumas.c:4777-4949 (no magnetic field). */
+        PG_NOINLINE int transport_with_csda(
+            Track & t, int material, double density) const
+        {
+                const double ki = t.energy;
+                const double di = t.distance;
+                const double ti = t.time;
+                const double xi = cel_grammage(p, MODE_CSDA, material, ki);
+                const double Ti = cel_proper_time(p, MODE_CSDA, material, ki);
+
+                int event = EVENT_NONE;
+                double xB;
+                if (c.event & EVENT_LIMIT_GRAMMAGE) {
+                        xB = c.limit_grammage - t.grammage;
+                        event = EVENT_LIMIT_GRAMMAGE;
+                } else
+                        xB = DBL_MAXV;
+                if (c.event & EVENT_LIMIT_DISTANCE) {
+                        const double xD = density * (c.limit_distance - di);
+                        if (xD < xB) {
+                                xB = xD;
+                                event = EVENT_LIMIT_DISTANCE;
+                        }
+                }
+                int decayed = 0;
+                double time_max =
+                    (c.event & EVENT_LIMIT_TIME) ? c.limit_time : 0.;
+                if (c.decay == MODE_RANDOMISED) {
+                        if ((time_max <= 0.) || (t.lifetime < time_max)) {
+                                time_max = t.lifetime;
+                                decayed = 1;
+                        }
+                }
+                if (time_max > 0.) {
+                        const double dt = time_max - ti;
+                        if (dt <= 0.) {
+                                xB = 0.;
+                                event = EVENT_LIMIT_TIME;
+                        } else {
+                                const double s = sgn();
+                                const double Tf = Ti - s * dt * density;
+                                if (Tf > 0.) {
+                                        const double xT = fabs(xi -
+                                            cel_grammage_as_time(
+                                                p, MODE_CSDA, material, Tf));
+                                        if (xT < xB) {
+                                                xB = xT;
+                                                event = EVENT_LIMIT_TIME;
+                                        }
+                                }
+                        }
+                }
+                if (xB <= 0.) return event;
+
+                double xf, kf;
+                if (c.direction == MODE_FORWARD) {
+                        if (xi > xB) {
+                                xf = xi - xB;
+                                if (xB < EPSILON_X * xi) {
+                                        kf = ki - cel_energy_loss(p, MODE_CSDA,
+                                                      material, ki) * xB;
+                                        if (kf < 0.) kf = 0.;
+                                } else {
+                                        kf = cel_kinetic_energy(
+                                            p, MODE_CSDA, material, xf);
+                                }
+                        } else {
+                                xf = kf = 0.;
+                                event = EVENT_LIMIT_ENERGY;
+                        }
+                        if (c.event & EVENT_LIMIT_ENERGY) {
+                                if (ki <= c.limit_energy)
+                                        return EVENT_LIMIT_ENERGY;
+                                if (kf < c.limit_energy) {
+                                        kf = c.limit_energy;
+                                        xf = cel_grammage(
+                                            p, MODE_CSDA, material, kf);
+                                        event = EVENT_LIMIT_ENERGY;
+                                }
+                        }
+                } else {
+                        if (xB == DBL_MAXV) {
+                                xf = xB;
+                                kf = DBL_MAXV;
+                        } else {
+                                xf = xB + xi;
+                                if (xB < EPSILON_X * xi) {
+                                        kf = ki + cel_energy_loss(p, MODE_CSDA,
+                                                      material, ki) * xB;
+                                } else {
+                                        kf = cel_kinetic_energy(
+                                            p, MODE_CSDA, material, xf);
+                                }
+                        }
+                        if (c.event & EVENT_LIMIT_ENERGY) {
+                                if (ki >= c.limit_energy)
+                                        return EVENT_LIMIT_ENERGY;
+                                if (kf > c.limit_energy) {
+                                        kf = c.limit_energy;
+                                        xf = cel_grammage(
+                                            p, MODE_CSDA, material, kf);
+                                        event = EVENT_LIMIT_ENERGY;
+                                }
+                        }
+                }
+
+                const double distance = fabs(xf - xi) / density;
+                t.energy = kf;
+                if (event == EVENT_LIMIT_DISTANCE)
+                        t.distance = c.limit_distance;
+                else
+                        t.distance += distance;
+                if (event == EVENT_LIMIT_GRAMMAGE)
+                        t.grammage = c.limit_grammage;
+                else
+                        t.grammage += distance * density;
+                if (event == EVENT_LIMIT_TIME) {
+                        t.time = time_max;
+                        t.decayed = decayed;
+                        if (decayed) event = EVENT_VERTEX_DECAY;
+                } else
+                        t.time += fabs(Ti - cel_proper_time(p, MODE_CSDA,
+                                                material, kf)) / density;
+                if (c.direction == MODE_BACKWARD)
+                        t.weight *=
+                            cel_energy_loss(p, MODE_CSDA, material, kf) /
+                            cel_energy_loss(p, MODE_CSDA, material, ki);
+                if (c.decay == MODE_WEIGHTED)
+                        t.weight *= exp(-fabs(ti - t.time) / p.ctau);
+
+                const double path =
+                    (c.direction == MODE_FORWARD) ? distance : -distance;
+                t.pos[0] += path * t.dir[0];
+                t.pos[1] += path * t.dir[1];
+                t.pos[2] += path * t.dir[2];
+                return event;
+        }
+
+        /* transport_limit, src/pumas.c:5569-5696. */
+        PG_FN void transport_limit(Track & t, int material, double Xi,
+            double Xf, double * grammage_max) const
+        {
+                t.step_event = t.step_foreseen = EVENT_NONE;
+                if (t.weight <= 0.) {
+                        t.step_event = EVENT_WEIGHT;
+                        return;
+                }
+                if (c.direction == MODE_FORWARD) {
+                        const double kinetic_min =
+                            (c.event & EVENT_LIMIT_ENERGY) ? c.limit_energy :
+                                                             0.;
+                        if (t.energy <= kinetic_min) {
+                                t.step_event = EVENT_LIMIT_ENERGY;
+                                return;
+                        }
+                } else if ((c.event & EVENT_LIMIT_ENERGY) &&
+                    (t.energy >= c.limit_energy)) {
+                        t.step_event = EVENT_LIMIT_ENERGY;
+                        return;
+                }
+
+                if (c.event & EVENT_LIMIT_GRAMMAGE) {
+                        *grammage_max = c.limit_grammage;
+                        t.step_foreseen = EVENT_LIMIT_GRAMMAGE;
+                } else {
+                        *grammage_max = 0.;
+                }
+
+                const int scheme = c.energy_loss;
+                if (scheme == MODE_DISABLED) {
+                        if (c.scattering == MODE_MIXED) {
+                                const double X = Xi -
+                                    coulomb_ehs_length(p, material, t.energy) *
+                                        log(t.rng.uniform());
+                                if ((*grammage_max == 0.) ||
+                                    (X < *grammage_max)) {
+                                        *grammage_max = X;
+                                        t.step_foreseen = EVENT_VERTEX_COULOMB;
+                                        return;
+                                }
+                        }
+                        return;
+                }
+
+                const double s = sgn();
+                int foreseen = EVENT_NONE;
+                double kinetic_limit = 0.;
+                if (scheme == MODE_MIXED) {
+                        for (int i = 0; i < 1000; i++) {
+                                const double zeta = t.rng.uniform();
+                                if ((zeta <= 0.) || (zeta >= 1.)) continue;
+                                const double nI = del_interaction_length(
+                                    p, material, t.energy) + s * log(zeta);
+                                if (nI <= 0.) break;
+                                const double k =
+                                    del_kinetic_from_interaction_length(
+                                        p, material, nI);
+                                if (c.direction == MODE_FORWARD) {
+                                        if (k >= t.energy) continue;
+                                } else {
+                                        if (k <= t.energy) continue;
+                                }
+                                if ((c.direction == MODE_BACKWARD) ||
+                                    (k > p.Kt[material])) {
+                                        kinetic_limit = k;
+                                        foreseen = EVENT_VERTEX_DEL;
+                                }
+                                break;
+                        }
+                }
+
+                if ((scheme < MODE_STRAGGLED) &&
+                    (c.scattering == MODE_MIXED)) {
+                        const double nI = ehs_interaction_length(
+                            p, scheme, material, t.energy) +
+                            s * log(t.rng.uniform());
+                        if (nI > 0.) {
+                                const double k =
+                                    ehs_kinetic_from_interaction_length(
+                                        p, scheme, material, nI);
+                                if ((kinetic_limit <= 0.) ||
+                                    ((c.direction == MODE_FORWARD) &&
+                                        (k > kinetic_limit)) ||
+                                    ((c.direction == MODE_BACKWARD) &&
+                                        (k < kinetic_limit))) {
+                                        kinetic_limit = k;
+                                        foreseen = EVENT_VERTEX_COULOMB;
+                                }
+                        }
+                }
+
+                if (foreseen == EVENT_NONE) return;
+
+                const double X = Xi +
+                    s * (Xf - cel_grammage(p, scheme, material, kinetic_limit));
+                if ((*grammage_max <= 0) || (X < *grammage_max)) {
+                        *grammage_max = X;
+                        t.step_foreseen = foreseen;
+                }
+        }
+
+        /* step_fluctuate, src/pumas.c:6943-7005. */
+        PG_FN void step_fluctuate(Track & t, int material, double Xtot,
+            double dX, double * kf, double * ratio) const
+        {
+                const int scheme = c.energy_loss;
+                const double s = sgn();
+                double k1, r = 0.;
+                k1 = cel_kinetic_energy(p, scheme, material, Xtot - s * dX);
+                const double dk0 = fabs(t.energy - k1);
+                if ((k1 > 0.) && (dk0 > 0.)) {
+                        const double Omega0 =
+                            cel_straggling(p, material, t.energy);
+                        const double Omega1 = cel_straggling(p, material, k1);
+                        double dk12 = 0.5 * dX * (Omega0 + Omega1);
+                        const double tmp = dX / Xtot;
+                        if (tmp > 5E-02) {
+                                const double de1 =
+                                    cel_energy_loss(p, scheme, material, k1);
+                                const double de0 = cel_energy_loss(
+                                    p, scheme, material, t.energy);
+                                dk12 *= 1. + s * dX / dk0 * (de1 - de0);
+                        }
+                        const double dk1 = sqrt(dk12);
+                        if (dk0 >= 3. * dk1) {
+                                double u;
+                                do
+                                        u = randn(t);
+                                while (fabs(u) > 3.);
+                                u /= 1.015387;
+                                k1 += u * dk1;
+                        } else if (dk0 >= 1.7320508 * dk1) {
+                                const double u =
+                                    1.7320508 * (1. - 2. * t.rng.uniform());
+                                k1 += u * dk1;
+                        } else {
+                                const double dk32 = 3. * dk12;
+                                const double dk02 = dk0 * dk0;
+                                const double a =
+                                    1. - (dk32 - dk02) / (dk32 + 3. * dk02);
+                                if (t.rng.uniform() <= a) {
+                                        const double b =
+                                            0.5 * (dk32 + 3. * dk02) / dk0;
+                                        const double u = t.rng.uniform();
+                                        k1 = t.energy - s * b * u;
+                                        if (k1 < 0.) k1 = 0.;
+                                } else {
+                                        k1 = t.energy;
+                                }
+                        }
+                        r = fabs(t.energy - k1) / dk0;
+                }
+                if (r == 0.) r = 1.;
+                *kf = k1;
+                *ratio = r;
+        }
+
+        /* step_transport, src/pumas.c:6296-6925 (uniform media, no magnetic
+         * field). Returns the end medium through `out_medium`. */
+        PG_NOINLINE void step_transport(Track & t, int straight, int medium,
+            int material, double density, double grammage_max,
+            double step_max_medium, int step_max_type, int * out_medium) const
+        {
+                const int scheme = c.energy_loss;
+                double * const direction = t.dir;
+                double * const position = t.pos;
+                const double density_i = 1. / density;
+                const double locals_density = density;
+                const double momentum =
+                    sqrt(t.energy * (t.energy + 2. * p.mass));
+
+                const int tmp_scheme =
+                    (scheme == MODE_DISABLED) ? MODE_CSDA : scheme;
+                const double Xtot =
+                    cel_grammage(p, tmp_scheme, material, t.energy);
+
+                double step_loc;
+                double invlb1 = 0.;
+                if (!straight) {
+                        double r = c.accuracy;
+                        const double k_threshold = 1E+09;
+                        if ((scheme == MODE_STRAGGLED) &&
+                            (t.energy > k_threshold)) {
+                                double f = k_threshold / t.energy;
+                                if (f < 0.1) f = 0.1;
+                                r *= f;
+                        }
+                        step_loc = r * density_i * Xtot;
+
+                        if (c.scattering == MODE_MIXED) {
+                                if (t.step_first != 0) {
+                                        double mu0;
+                                        table_get_msc(p, scheme, material,
+                                            t.energy, &mu0, &invlb1);
+                                        invlb1 *= density;
+                                } else
+                                        invlb1 = t.step_invlb1;
+                                const double stepT = c.accuracy / invlb1;
+                                if (stepT < step_loc) step_loc = stepT;
+                        }
+                } else {
+                        if (grammage_max <= 0.)
+                                step_loc = 1E+09 * density_i;
+                        else
+                                step_loc = grammage_max * density_i;
+                }
+
+                t.step_first = 0;
+
+                if (step_loc < STEP_MIN) step_loc = STEP_MIN;
+                double step = (step_max_medium <= 0) ?
+                    step_loc :
+                    (step_max_medium < step_loc ? step_max_medium : step_loc);
+
+                *out_medium = medium;
+                int event = EVENT_NONE;
+                if (c.event & EVENT_LIMIT_DISTANCE) {
+                        const double d = c.limit_distance - t.distance;
+                        if (d <= step) {
+                                step = d;
+                                event = EVENT_LIMIT_DISTANCE;
+                        }
+                }
+
+                const double sg = sgn();
+                position[0] += step * sg * direction[0];
+                position[1] += step * sg * direction[1];
+                position[2] += step * sg * direction[2];
+
+                int end_medium = medium_at(position, t);
+                double end_position[3] = { position[0], position[1],
+                        position[2] };
+                if (end_medium != medium) {
+                        if (step_max_type == STEP_CHECK) {
+                                const double step_min =
+                                    (step < STEP_MIN) ? step : STEP_MIN;
+                                double pi[3] = { position[0], position[1],
+                                        position[2] };
+                                double s1 = 0., s2 = -step_min;
+                                position[0] = pi[0] + s2 * sg * direction[0];
+                                position[1] = pi[1] + s2 * sg * direction[1];
+                                position[2] = pi[2] + s2 * sg * direction[2];
+                                int tmp_medium = medium_at(position, t);
+                                if (tmp_medium != medium) {
+                                        if (tmp_medium != end_medium)
+                                                end_medium = tmp_medium;
+                                        s1 = s2;
+                                        s2 = -step;
+                                        while (fabs(s1 - s2) > STEP_MIN) {
+                                                const double s3 =
+                                                    0.5 * (s1 + s2);
+                                                position[0] = pi[0] +
+                                                    s3 * sg * direction[0];
+                                                position[1] = pi[1] +
+                                                    s3 * sg * direction[1];
+                                                position[2] = pi[2] +
+                                                    s3 * sg * direction[2];
+                                                tmp_medium =
+                                                    medium_at(position, t);
+                                                if (tmp_medium == medium) {
+                                                        s2 = s3;
+                                                } else {
+                                                        s1 = s3;
+                                                        if (tmp_medium !=
+                                                            end_medium)
+                                                                end_medium =
+                                                                    tmp_medium;
+                                                }
+                                        }
+                                        position[0] =
+                                            pi[0] + s2 * sg * direction[0];
+                                        position[1] =
+                                            pi[1] + s2 * sg * direction[1];
+                                        position[2] =
+                                            pi[2] + s2 * sg * direction[2];
+                                        step += s1;
+                                        end_position[0] =
+                                            pi[0] + s1 * sg * direction[0];
+                                        end_position[1] =
+                                            pi[1] + s1 * sg * direction[1];
+                                        end_position[2] =
+                                            pi[2] + s1 * sg * direction[2];
+                                }
+                        }
+                        event = EVENT_MEDIUM;
+                        *out_medium = end_medium;
+                }
+
+                if (event & EVENT_MEDIUM) {
+                        position[0] = end_position[0];
+                        position[1] = end_position[1];
+                        position[2] = end_position[2];
+                }
+
+                double k1 = t.energy, dk = 0.;
+                const double dX = 0.5 * step * (density + locals_density);
+                if ((scheme >= MODE_CSDA) && (scheme <= MODE_MIXED)) {
+                        const double X = Xtot - sg * dX;
+                        if ((c.direction == MODE_FORWARD) &&
+                            (X <= t.step_X_limit)) {
+                                k1 = (c.event & EVENT_LIMIT_ENERGY) ?
+                                    c.limit_energy :
+                                    0.;
+                                const double grammage =
+                                    t.grammage + Xtot - t.step_X_limit;
+                                if ((grammage_max <= 0.) ||
+                                    (grammage < grammage_max)) {
+                                        grammage_max = grammage;
+                                        event = EVENT_LIMIT_ENERGY;
+                                }
+                        } else if ((c.direction == MODE_BACKWARD) &&
+                            (t.step_X_limit > 0.) && (X > t.step_X_limit)) {
+                                k1 = (c.event & EVENT_LIMIT_ENERGY) ?
+                                    c.limit_energy :
+                                    0.;
+                                const double grammage =
+                                    t.grammage + t.step_X_limit - Xtot;
+                                if ((grammage_max <= 0.) ||
+                                    (grammage < grammage_max)) {
+                                        grammage_max = grammage;
+                                        event = EVENT_LIMIT_ENERGY;
+                                }
+                        } else if (dX < EPSILON_X * Xtot) {
+                                k1 -= sg *
+                                    cel_energy_loss(p, scheme, material, k1) *
+                                    dX;
+                                if ((c.event & EVENT_LIMIT_ENERGY) &&
+                                    (c.limit_energy > 0.)) {
+                                        if (((c.direction == MODE_FORWARD) &&
+                                                (k1 < c.limit_energy)) ||
+                                            ((c.direction == MODE_BACKWARD) &&
+                                                (k1 > c.limit_energy))) {
+                                                k1 = c.limit_energy;
+                                                event = EVENT_LIMIT_ENERGY;
+                                        }
+                                } else if (k1 < 0.) {
+                                        k1 = 0.;
+                                }
+                        } else {
+                                k1 = cel_kinetic_energy(
+                                    p, scheme, material, X);
+                        }
+                } else if (scheme == MODE_STRAGGLED) {
+                        double ratio;
+                        step_fluctuate(t, material, Xtot, dX, &k1, &ratio);
+                        dk = fabs(t.energy - k1);
+
+                        double kinetic_limit = -1.;
+                        if (c.direction == MODE_FORWARD) {
+                                const double kinetic_min =
+                                    (c.event & EVENT_LIMIT_ENERGY) ?
+                                    c.limit_energy :
+                                    0.;
+                                if (k1 <= kinetic_min)
+                                        kinetic_limit = kinetic_min;
+                        } else {
+                                if ((c.event & EVENT_LIMIT_ENERGY) &&
+                                    (k1 >= c.limit_energy))
+                                        kinetic_limit = c.limit_energy;
+                        }
+                        if (kinetic_limit >= 0.) {
+                                const double grammage = t.grammage +
+                                    fabs(t.energy - kinetic_limit) / dk * dX;
+                                k1 = kinetic_limit;
+                                if ((grammage_max <= 0.) ||
+                                    (grammage < grammage_max)) {
+                                        grammage_max = grammage;
+                                        event = EVENT_LIMIT_ENERGY;
+                                }
+                        }
+
+                        double Xmax = dX;
+                        if (grammage_max > 0.) {
+                                const double dX1 = grammage_max - t.grammage;
+                                if (Xmax > dX1) Xmax = dX1;
+                        }
+                        double k_x = -1.;
+
+                        /* In-step inelastic DEL. */
+                        do {
+                                double xs_del =
+                                    del_cross_section(p, material, t.energy);
+                                const double tmp_del =
+                                    del_cross_section(p, material, k1);
+                                if (tmp_del > xs_del) xs_del = tmp_del;
+                                if (xs_del <= 0.) break;
+                                const double X_del =
+                                    -log(t.rng.uniform()) / xs_del;
+                                if (X_del >= Xmax) break;
+                                const double k_h = cel_kinetic_energy(
+                                    p, scheme, material, Xtot - sg * X_del);
+                                const double k_del = t.energy -
+                                    sg * ratio * fabs(t.energy - k_h);
+                                if (k_del <= 0.) break;
+                                const double r =
+                                    del_cross_section(p, material, k_del) /
+                                    xs_del;
+                                if (t.rng.uniform() > r) break;
+                                k_x = k_del;
+                                Xmax = X_del;
+                                event = EVENT_VERTEX_DEL;
+                        } while (0);
+
+                        /* In-step elastic hard scattering. */
+                        if (c.scattering == MODE_MIXED) do {
+                                double kmin, kmax;
+                                if (k1 <= t.energy)
+                                        kmin = k1, kmax = t.energy;
+                                else
+                                        kmax = k1, kmin = t.energy;
+                                if (kmin < p.K[1]) {
+                                        const double tmp = kmin;
+                                        kmin = kmax, kmax = tmp;
+                                }
+                                const double lb_ehs =
+                                    coulomb_ehs_length(p, material, kmin);
+                                if (lb_ehs <= 0.) break;
+                                const double X_ehs =
+                                    -log(t.rng.uniform()) * lb_ehs;
+                                if (X_ehs >= Xmax) break;
+                                const double k_h = cel_kinetic_energy(
+                                    p, scheme, material, Xtot - sg * X_ehs);
+                                const double k_ehs = t.energy -
+                                    sg * ratio * fabs(t.energy - k_h);
+                                if (k_ehs <= 0.) break;
+                                const double r =
+                                    coulomb_ehs_length(p, material, k_ehs) /
+                                    lb_ehs;
+                                if ((r <= 0.) || (t.rng.uniform() > 1. / r))
+                                        break;
+                                k_x = k_ehs;
+                                Xmax = X_ehs;
+                                event = EVENT_VERTEX_COULOMB;
+                        } while (0);
+
+                        if (k_x >= 0.) {
+                                k1 = k_x;
+                                grammage_max = t.grammage + Xmax;
+                        }
+                }
+
+                const double Xi = t.grammage;
+                t.grammage += dX;
+                const double sf0 = step;
+                if ((grammage_max > 0.) && (t.grammage >= grammage_max)) {
+                        const double dX_ = grammage_max - Xi;
+                        if ((fabs(density - locals_density) <= FLT_EPS) ||
+                            (sf0 <= FLT_EPS)) {
+                                step = dX_ * density_i;
+                        } else {
+                                const double drho = locals_density - density;
+                                double tmp =
+                                    density * density + 2. * dX_ * drho / sf0;
+                                tmp = (tmp > FLT_EPS) ? sqrt(tmp) : 0;
+                                const double h_int = (tmp - density) / drho;
+                                step *= h_int;
+                        }
+                        t.grammage = grammage_max;
+                        if (!(event & EVENT_LIMIT_ENERGY)) {
+                                if (scheme <= MODE_MIXED) {
+                                        if (scheme != MODE_DISABLED)
+                                                k1 = cel_kinetic_energy(p,
+                                                    scheme, material,
+                                                    Xtot - sg * (t.grammage -
+                                                                    Xi));
+                                        event = t.step_foreseen;
+                                } else if (!(event & (EVENT_VERTEX_COULOMB |
+                                                         EVENT_VERTEX_DEL))) {
+                                        k1 = t.energy -
+                                            sg * (t.grammage - Xi) * dk / dX;
+                                        if (k1 < 0.) k1 = 0.;
+                                        event = EVENT_LIMIT_GRAMMAGE;
+                                }
+                        }
+
+                        const double ds_ = step - sf0;
+                        position[0] += ds_ * sg * direction[0];
+                        position[1] += ds_ * sg * direction[1];
+                        position[2] += ds_ * sg * direction[2];
+                }
+
+                int decayed = 0;
+                double time_max =
+                    (c.event & EVENT_LIMIT_TIME) ? c.limit_time : 0.;
+                if (c.decay == MODE_RANDOMISED) {
+                        if ((time_max <= 0.) || (t.lifetime < time_max)) {
+                                time_max = t.lifetime;
+                                decayed = 1;
+                        }
+                }
+
+                const double sf1 = step;
+                if (straight && (scheme != MODE_DISABLED)) {
+                        const double Ti =
+                            cel_proper_time(p, scheme, material, t.energy);
+                        if (time_max > 0.) {
+                                const double Tf = Ti -
+                                    sg * (time_max - t.time) * density;
+                                if (Tf > 0.) {
+                                        const double dxT = fabs(Xtot -
+                                            cel_grammage_as_time(
+                                                p, scheme, material, Tf));
+                                        if (Xi + dxT < t.grammage) {
+                                                event = EVENT_LIMIT_TIME;
+                                                t.time = time_max;
+                                                t.decayed = decayed;
+                                                step = dxT * density_i;
+                                                if (step > sf1) step = sf1;
+                                                t.grammage = Xi + dxT;
+                                                const double xf =
+                                                    Xtot - sg * dxT;
+                                                k1 = cel_kinetic_energy(
+                                                    p, scheme, material, xf);
+                                        }
+                                }
+                        }
+                        if (event != EVENT_LIMIT_TIME) {
+                                const double Tf =
+                                    cel_proper_time(p, scheme, material, k1);
+                                t.time += fabs(Tf - Ti) * density_i;
+                        }
+                } else {
+                        const double p_f = (k1 <= 0) ?
+                            momentum :
+                            sqrt(k1 * (k1 + 2. * p.mass));
+                        const double ti = t.time;
+                        t.time +=
+                            0.5 * step * p.mass * (1. / momentum + 1. / p_f);
+                        if ((time_max > 0.) && (t.time >= time_max)) {
+                                event = EVENT_LIMIT_TIME;
+                                t.time = time_max;
+                                t.decayed = decayed;
+
+                                const double a = (momentum / p_f - 1.) / sf1;
+                                const double cc =
+                                    -2. * (time_max - ti) * momentum / p.mass;
+                                if (a != 0.) {
+                                        double delta = 1. - a * cc;
+                                        if (delta < 0.) delta = 0.;
+                                        step = 1. / a * (sqrt(delta) - 1.);
+                                } else {
+                                        step = -0.5 * cc;
+                                }
+                                if (step < 0.) step = 0.;
+
+                                if (scheme != MODE_DISABLED) {
+                                        const double p1 =
+                                            momentum / (1 + a * step);
+                                        k1 = sqrt(p1 * p1 - p.mass * p.mass) -
+                                            p.mass;
+                                        if (k1 < 0.) k1 = 0.;
+                                }
+
+                                const double Xf = Xi + dX;
+                                if (density == locals_density)
+                                        t.grammage = Xi + step * density;
+                                else
+                                        t.grammage = Xi +
+                                            step * (density + 0.5 * step /
+                                                sf0 * (locals_density -
+                                                          density));
+                                if (t.grammage > Xf) t.grammage = Xf;
+                        }
+                }
+
+                if (event == EVENT_LIMIT_TIME) {
+                        const double ds_ = step - sf1;
+                        position[0] += ds_ * sg * direction[0];
+                        position[1] += ds_ * sg * direction[1];
+                        position[2] += ds_ * sg * direction[2];
+                }
+
+                t.step_event = event;
+                t.energy = k1;
+                t.distance += step;
+
+                if (c.scattering == MODE_MIXED) {
+                        double mu0, invlb1_;
+                        if (t.energy <= 0.) {
+                                t.step_invlb1 = invlb1;
+                        } else {
+                                table_get_msc(p, scheme, material, t.energy,
+                                    &mu0, &invlb1_);
+                                t.step_invlb1 = locals_density * invlb1_;
+                        }
+                }
+
+                if ((invlb1 > 0.) || (t.step_invlb1 > 0.)) {
+                        double ilb1 = 0.25 * step * (invlb1 + t.step_invlb1);
+                        if (ilb1 > 1.) ilb1 = 1.;
+                        double mu;
+                        do {
+                                mu = -ilb1 * log(t.rng.uniform());
+                        } while (mu > 1.);
+                        rotate(t, mu);
+                }
+        }
+
+        /* Position of a material's first component in the global list. */
+        PG_FN int component0(int material) const
+        {
+                return p.material_component0[material];
+        }
+
+        /*
+         * del_randomise_target, src/pumas.c:6154-6270. Returns the process
+         * and sets the element (global index) and, backward, the weight
+         * (`weight` holds Q on input in backward mode).
+         */
+        PG_FN void del_randomise_target(Track & t, int material,
+            double * weight, int * process, int * element) const
+        {
+                int i1, i2;
+                double h, dk;
+                i1 = table_index(p.K, p.n_energies, t.energy);
+                if (i1 < 0) {
+                        i1 = i2 = 0;
+                        h = dk = 0.;
+                } else if (i1 >= p.n_energies - 1) {
+                        i1 = i2 = p.n_energies - 1;
+                        h = dk = 0.;
+                } else {
+                        i2 = i1 + 1;
+                        const double K1 = p.K[i1];
+                        dk = p.K[i2] - K1;
+                        h = (t.energy - K1) / dk;
+                }
+                (void)i2;
+
+                const int ic0 = component0(material);
+                const int nc = p.elements_in[material];
+                if (c.direction == MODE_FORWARD) {
+                        const double zeta = t.rng.uniform();
+                        for (int ic = ic0; ic < ic0 + nc; ic++)
+                                for (int ip = 0; ip < N_DEL_PROCESSES; ip++) {
+                                        const double * f = p.CSf +
+                                            (ip * p.n_components + ic) *
+                                                p.n_energies + i1;
+                                        const double * df = p.CSf_dK +
+                                            (ip * p.n_components + ic) *
+                                                p.n_energies + i1;
+                                        const double csf = pchip(h, f[0], f[1],
+                                            df[0] * dk, df[1] * dk);
+                                        if (!(zeta > csf)) {
+                                                const int iel =
+                                                    p.component_element[ic];
+                                                const double * n = p.CSn +
+                                                    (ip * p.n_elements + iel) *
+                                                        p.n_energies + i1;
+                                                const double * dn = p.CSn_dK +
+                                                    (ip * p.n_elements + iel) *
+                                                        p.n_energies + i1;
+                                                const double csn = pchip(h,
+                                                    n[0], n[1], dn[0] * dk,
+                                                    dn[1] * dk);
+                                                *weight = 1. / csn;
+                                                *element = iel;
+                                                *process = ip;
+                                                return;
+                                        }
+                                }
+                        /* Not reachable for normalised fractions. */
+                        *element = p.component_element[ic0 + nc - 1];
+                        *process = N_DEL_PROCESSES - 1;
+                        return;
+                } else {
+                        const double Q = *weight;
+                        double stot = 0.;
+                        for (int ip = 0; ip < N_DEL_PROCESSES; ip++) {
+                                for (int ic = ic0; ic < ic0 + nc; ic++) {
+                                        const int iel = p.component_element[ic];
+                                        const double d = dcs_evaluate(
+                                            p, mv, ip, iel, t.energy, Q);
+                                        stot += d * p.component_fraction[ic];
+                                }
+                        }
+                        if (stot == 0.) {
+                                *weight = 0.;
+                                return;
+                        } else {
+                                *weight = stot;
+                        }
+
+                        if (c.scattering == MODE_DISABLED) {
+                                *element = p.n_elements;
+                                *process = N_DEL_PROCESSES;
+                                return;
+                        }
+
+                        double zeta;
+                        for (;;) {
+                                zeta = t.rng.uniform() * stot;
+                                if ((zeta > 0.) && (zeta < stot)) break;
+                        }
+                        double s = 0.;
+                        for (int ic = ic0; ic < ic0 + nc; ic++)
+                                for (int ip = 0; ip < N_DEL_PROCESSES; ip++) {
+                                        const int iel = p.component_element[ic];
+                                        const double si = dcs_evaluate(p, mv,
+                                            ip, iel, t.energy, Q) *
+                                            p.component_fraction[ic];
+                                        s += si;
+                                        if (!(zeta > s)) {
+                                                *element = iel;
+                                                *process = ip;
+                                                return;
+                                        }
+                                }
+                        *element = p.component_element[ic0 + nc - 1];
+                        *process = N_DEL_PROCESSES - 1;
+                }
+        }
+
+        /* dcs_ionisation_randomise, src/pumas.c:11185-11226. */
+        PG_FN double ionisation_randomise(
+            Track & t, int iel, double K, double xlow) const
+        {
+                const double P2 = K * (K + 2. * p.mass);
+                const double E = K + p.mass;
+                const double Wmax = 2. * ELECTRON_MASS * P2 /
+                    (p.mass * p.mass +
+                        ELECTRON_MASS * (ELECTRON_MASS + 2. * E));
+                if (Wmax < p.cutoff * K) return K;
+                double Wmin = (13.6 / 19.2) * p.element_I[iel];
+                const double qlow = K * xlow;
+                if (qlow >= Wmin) Wmin = qlow;
+
+                const double E2 = E * E;
+                const double beta2 = P2 / E2;
+                const double a0 = 0.5 / E2;
+                const double a1 = beta2 / Wmax;
+
+                const double p0 = a0 * (Wmax - Wmin);
+                const double p2 = 1. / Wmin - 1. / Wmax;
+                double q;
+                for (;;) {
+                        double z = t.rng.uniform() * (p0 + p2);
+                        if (z <= p0) {
+                                z /= p0;
+                                q = (Wmax - Wmin) * z;
+                        } else {
+                                z = (z - p0) / p2;
+                                q = Wmin * Wmax / (Wmax - z * (Wmax - Wmin));
+                        }
+                        const double r0 = a0 + 1 / (q * q);
+                        const double r1 = r0 - a1 / q;
+                        if (t.rng.uniform() * r0 <= r1) break;
+                }
+                return K - q;
+        }
+
+        /* del_randomise_forward, src/pumas.c:5912-6017. Sets the process
+         * and element once a target is drawn; returns 1 when a polar angle
+         * is to be sampled (a non NULL polar function in the reference). */
+        PG_NOINLINE int del_randomise_forward(
+            Track & t, int material, int * process_out, int * element) const
+        {
+                if (t.energy <= p.Kt[material]) return 0;
+
+                double weight;
+                int process, iel;
+                del_randomise_target(t, material, &weight, &process, &iel);
+                *element = iel;
+                *process_out = process;
+
+                if (process == 3) {
+                        t.energy =
+                            ionisation_randomise(t, iel, t.energy, p.cutoff);
+                        return 1;
+                }
+
+                double qmin, qmax;
+                if (process == 2) {
+                        dcs_photonuclear_range(p.mass, t.energy, &qmin, &qmax);
+                } else {
+                        dcs_pair_production_range(
+                            mv, iel, p.mass, t.energy, &qmin, &qmax);
+                        if (process == 0) qmin = 0.;
+                }
+                double xmin = qmin / t.energy;
+                if (xmin < p.cutoff) xmin = p.cutoff;
+                const double xmax = qmax / t.energy;
+                if (xmin >= xmax) return 0;
+
+                const int eidx = p.element_index[iel];
+                double a0, a1, b0, b1, p0;
+                const int imax = p.n_energies - 1;
+                if (t.energy >= p.K[imax]) {
+                        const float * envelope = p.DCS_envelope +
+                            2 * ((process * p.n_elements + eidx) *
+                                        p.n_energies + imax);
+                        a0 = (double)envelope[0];
+                        b0 = (double)envelope[1];
+                        p0 = 1.;
+                        a1 = b1 = 0.;
+                } else {
+                        const int row = table_index(p.K, p.n_energies, t.energy);
+                        const float * envelope0 = p.DCS_envelope +
+                            2 * ((process * p.n_elements + eidx) *
+                                        p.n_energies + row);
+                        const float * envelope1 = envelope0 + 2;
+                        a0 = (double)envelope0[0];
+                        b0 = (double)envelope0[1];
+                        a1 = (double)envelope1[0];
+                        b1 = (double)envelope1[1];
+
+                        const double k0 = p.K[row];
+                        const double k1 = p.K[row + 1];
+                        const double h1 = (t.energy - k0) / (k1 - k0);
+                        const double h0 = 1. - h1;
+                        b0 *= h0;
+                        b1 *= h1;
+
+                        p0 = envelope_norm(a0, xmin, xmax) * b0;
+                        const double pp1 = envelope_norm(a1, xmin, xmax) * b1;
+                        p0 /= p0 + pp1;
+                }
+
+                double x = 0.;
+                int i;
+                for (i = 0; i < 100; i++) {
+                        double u = t.rng.uniform();
+                        double a;
+                        if (u < p0) {
+                                a = a0;
+                                u = u / p0;
+                        } else {
+                                a = a1;
+                                u = (u - p0) / (1. - p0);
+                        }
+                        if (a == 1) {
+                                x = xmin * exp(u * log(xmax / xmin));
+                        } else {
+                                const double am = a - 1.;
+                                const double pmin = pow(xmin, -am);
+                                const double pmax = pow(xmax, -am);
+                                x = pow(pmin - u * (pmin - pmax), -1. / am);
+                        }
+
+                        const double penv =
+                            b0 * pow(x, -a0) + b1 * pow(x, -a1);
+                        const double d = dcs_evaluate(
+                            p, mv, process, iel, t.energy, t.energy * x);
+                        if (t.rng.uniform() * penv <= d) break;
+                }
+                if (i == 100) return 0;
+                t.energy *= (1. - x);
+                return 1;
+        }
+
+        /* del_randomise_power_law, src/pumas.c:6109-6138 (alpha != 1). */
+        PG_FN void del_randomise_power_law(Track & t, double alpha,
+            double xmin, double xmax, double * p_r, double * p_w) const
+        {
+                double r = 0., w_bias;
+                const double a1 = 1. - alpha;
+                const double x0 = pow(xmin, a1);
+                const double x1 = pow(xmax, a1);
+                for (;;) {
+                        const double z = t.rng.uniform();
+                        const double tmp = x0 + z * (x1 - x0);
+                        r = pow(tmp, 1. / a1);
+                        if ((r < xmin) || (r >= xmax)) continue;
+                        w_bias = (x1 - x0) * r / (a1 * tmp);
+                        break;
+                }
+                *p_r = 1. - r;
+                *p_w = w_bias;
+        }
+
+        /* del_randomise_reverse, src/pumas.c:6036-6093. Returns the process
+         * index with a polar angle to sample, -1 otherwise. */
+        PG_NOINLINE int del_randomise_reverse(
+            Track & t, int material, int * process_out, int * element) const
+        {
+                const double lnq0 = -log(p.cutoff);
+                const double kt = p.Kt[material];
+                const double kf = t.energy;
+                const double pCEL =
+                    (kf >= kt) ? 0. : lnq0 / (lnq0 + log(kt / (kt - kf)));
+                if (pCEL && (t.rng.uniform() < pCEL)) {
+                        t.weight /= pCEL;
+                        return -1;
+                }
+                const double xf = (kf >= kt * (1. - p.cutoff)) ?
+                    p.cutoff :
+                    1. - kf / kt;
+
+                double xmax;
+                const double m1 = p.mass - ELECTRON_MASS;
+                if (t.energy < 0.5 * m1 * m1 / ELECTRON_MASS) {
+                        const double m2 = p.mass + ELECTRON_MASS;
+                        xmax = 2. * ELECTRON_MASS * (t.energy + 2. * p.mass) /
+                            (m2 * m2);
+                        if (xmax < xf)
+                                return -1;
+                        else if (xmax > 1.)
+                                xmax = 1.;
+                } else {
+                        xmax = 1.;
+                }
+
+                double r, w_bias;
+                del_randomise_power_law(t, BMC_ALPHA, xf, xmax, &r, &w_bias);
+                w_bias /= r;
+                t.energy /= r;
+
+                double weight = t.energy - kf;
+                int process, iel;
+                del_randomise_target(t, material, &weight, &process, &iel);
+                if (weight <= 0) {
+                        t.weight = 0.;
+                        return -1;
+                }
+
+                t.weight *= w_bias * weight /
+                    (del_cross_section(p, material, kf) * (1. - pCEL));
+                *process_out = process;
+
+                if (c.scattering == MODE_DISABLED) return -1;
+                *element = iel;
+                return process;
+        }
+
+        /* transport_do_del, src/pumas.c:5706-5757. */
+        PG_NOINLINE void transport_do_del(Track & t, int material) const
+        {
+                double ki, kf;
+                int process = -1, polar, iel = -1;
+                if (c.direction == MODE_FORWARD) {
+                        ki = t.energy;
+                        polar = del_randomise_forward(
+                                    t, material, &process, &iel) ?
+                            process :
+                            -1;
+                        kf = t.energy;
+                } else {
+                        kf = t.energy;
+                        polar = del_randomise_reverse(
+                            t, material, &process, &iel);
+                        ki = t.energy;
+                }
+
+                if (process < 0) {
+                        if (t.weight > 0)
+                                t.step_event = EVENT_NONE;
+                        else
+                                t.step_event = EVENT_WEIGHT;
+                } else {
+                        const int event_for[N_DEL_PROCESSES + 1] = {
+                                EVENT_VERTEX_BREMSSTRAHLUNG,
+                                EVENT_VERTEX_PAIR_CREATION,
+                                EVENT_VERTEX_PHOTONUCLEAR,
+                                EVENT_VERTEX_DELTA_RAY, EVENT_VERTEX
+                        };
+                        t.step_event = event_for[process];
+                }
+
+                if ((c.scattering == MODE_MIXED) && (polar >= 0)) {
+                        if (kf > ki) {
+                                const double tmp = kf;
+                                kf = ki;
+                                ki = tmp;
+                        }
+                        const double mu =
+                            polar_sample(p, t.rng, polar, iel, ki, kf);
+                        rotate(t, mu);
+                }
+        }
+
+        /* transport_do_ehs, src/pumas.c:5768-5885. */
+        PG_NOINLINE void transport_do_ehs(Track & t, int material) const
+        {
+                const double kinetic = t.energy;
+                double mu0 = 0., invlb1 = 0.;
+                table_get_msc(p, c.energy_loss, material, kinetic, &mu0,
+                    &invlb1);
+
+                const int nc = p.elements_in[material];
+                const int ic0 = component0(material);
+                CoulombData data[MAX_COMPONENTS];
+                double cs_tot = 0.;
+                for (int i = 0; i < nc; i++) {
+                        CoulombData & d = data[i];
+                        const int iel = p.component_element[ic0 + i];
+                        const double Z = p.element_Z[iel];
+                        const double A = p.element_A[iel];
+                        double kinetic0;
+                        coulomb_frame_parameters(
+                            Z, A, p.mass, kinetic, &kinetic0, d.fCM);
+                        d.fspin = coulomb_spin_factor(p.mass, kinetic);
+                        coulomb_screening_parameters(p, iel, kinetic, kinetic0,
+                            &d.n_parameters, d.amplitude, d.screening);
+                        coulomb_pole_reduction(d);
+                        d.normalisation = p.component_fraction[ic0 + i] /
+                            coulomb_normalisation(
+                                Z, A, p.mass, kinetic, kinetic0);
+                        d.cs_hard = d.normalisation *
+                            coulomb_restricted_cs(mu0, d);
+                        cs_tot += d.cs_hard;
+                }
+
+                const double cc0 = t.rng.uniform() * cs_tot;
+                int ihard = nc - 1;
+                double cs = 0.;
+                for (int i = 0; i < nc; i++) {
+                        cs += data[i].cs_hard;
+                        if (cs >= cc0) {
+                                ihard = i;
+                                break;
+                        }
+                }
+
+                const CoulombData & d = data[ihard];
+                const int n = d.n_parameters - 1;
+                double A = d.screening[0];
+                for (int i = 1; i < n; i++) {
+                        const double Ai = d.screening[i];
+                        if (Ai < A) A = Ai;
+                }
+                double mu1;
+                for (;;) {
+                        const double zeta = t.rng.uniform();
+                        const double tmp = 1. + A - zeta * (1 - mu0);
+                        if (tmp <= 0.) continue;
+                        mu1 = (A + mu0) * (A + 1.) / tmp - A;
+                        if (mu1 < mu0)
+                                mu1 = mu0;
+                        else if (mu1 > 1.)
+                                mu1 = 1;
+
+                        double ratio = 0.;
+                        for (int i = 0; i < n; i++) {
+                                ratio += d.amplitude[i] * (A + mu1) /
+                                    (d.screening[i] + mu1);
+                        }
+                        ratio *= coulomb_nuclear_form_factor(
+                            mu1, d.screening[n]);
+                        ratio *= ratio * (1. - d.fspin * mu1);
+                        if (t.rng.uniform() <= ratio) break;
+                }
+
+                const double gamma = d.fCM[0];
+                const double tau = d.fCM[1];
+                double mu;
+                if (mu1 > 1E-06) {
+                        const double a = gamma * (tau + 1. - 2. * mu1);
+                        const double ct_h =
+                            a / sqrt(4. * mu1 * (1. - mu1) + a * a);
+                        mu = 0.5 * (1. - ct_h);
+                } else {
+                        const double dd = gamma * (1. + tau);
+                        mu = mu1 / (dd * dd);
+                }
+                rotate(t, mu);
+        }
+
+        /* transport_with_stepping, src/pumas.c:5194-5514. */
+        PG_NOINLINE int transport_with_stepping(Track & t, int * medium_ptr,
+            double density, double step_max_medium, int step_max_type,
+            double step_max_locals) const
+        {
+                int medium = *medium_ptr;
+                int material;
+                density = locals_density(medium, &material);
+
+                const int scheme = c.energy_loss;
+                int straight = ((c.scattering == MODE_DISABLED) &&
+                                   (scheme <= MODE_MIXED) &&
+                                   (step_max_locals <= 0.)) ?
+                    1 :
+                    0;
+
+                t.step_event = EVENT_NONE;
+
+                double ti = t.time;
+                double wi = t.weight;
+                double Xi = t.grammage;
+                double dei, Xf;
+                if (scheme > MODE_DISABLED) {
+                        const double ki = t.energy;
+                        Xf = cel_grammage(p, scheme, material, ki);
+                        dei = 1. / cel_energy_loss(p, scheme, material, ki);
+                } else {
+                        Xf = dei = 0.;
+                }
+
+                if ((c.event & EVENT_LIMIT_DISTANCE) &&
+                    (t.distance >= c.limit_distance))
+                        return EVENT_LIMIT_DISTANCE;
+                if ((c.event & EVENT_LIMIT_GRAMMAGE) &&
+                    (t.grammage >= c.limit_grammage))
+                        return EVENT_LIMIT_GRAMMAGE;
+                if ((c.event & EVENT_LIMIT_TIME) && (t.time >= c.limit_time))
+                        return EVENT_LIMIT_TIME;
+                if (t.weight <= 0.) return EVENT_WEIGHT;
+
+                double grammage_max = 0.;
+                t.step_event = EVENT_NONE;
+                t.step_first = 1;
+                t.step_X_limit = (c.event & EVENT_LIMIT_ENERGY) ?
+                    cel_grammage(p,
+                        (scheme > MODE_DISABLED) ? scheme : MODE_CSDA,
+                        material, c.limit_energy) :
+                    0.;
+                t.step_invlb1 = 0;
+                transport_limit(t, material, Xi, Xf, &grammage_max);
+                if (t.step_event) return t.step_event;
+
+                int step_index = 1;
+                for (;;) {
+                        if (step_index > 1) {
+                                int unused;
+                                step_max_type =
+                                    medium_step(t, &unused, &step_max_medium);
+                                if ((step_max_medium > 0.) &&
+                                    (step_max_type == STEP_CHECK))
+                                        step_max_medium += 0.5 * STEP_MIN;
+                        }
+                        if ((int)t.steps >= t.max_steps) {
+                                /* Safety cap: end the history as a limit. */
+                                t.step_event = EVENT_LIMIT;
+                                break;
+                        }
+
+                        int new_medium = -1;
+                        step_transport(t, straight, medium, material, density,
+                            grammage_max, step_max_medium, step_max_type,
+                            &new_medium);
+                        step_index++;
+
+                        if (!t.step_event) continue;
+
+                        if ((c.direction == MODE_BACKWARD) &&
+                            (c.energy_loss >= MODE_CSDA)) {
+                                const double wt = (c.decay == MODE_WEIGHTED) ?
+                                    exp(-fabs(t.time - ti) / p.ctau) :
+                                    1.;
+                                t.weight = wi * wt * dei *
+                                    cel_energy_loss(
+                                        p, scheme, material, t.energy);
+                        } else if (c.decay == MODE_WEIGHTED) {
+                                t.weight = wi * exp(-fabs(t.time - ti) / p.ctau);
+                        }
+
+                        if (t.step_event & (EVENT_LIMIT | EVENT_WEIGHT |
+                                               EVENT_VERTEX_DECAY)) {
+                                break;
+                        } else if (t.step_event &
+                            (EVENT_VERTEX_DEL | EVENT_VERTEX_COULOMB)) {
+                                if (t.step_event & EVENT_VERTEX_DEL) {
+                                        transport_do_del(t, material);
+                                        if ((c.event & t.step_event) ||
+                                            t.step_event == EVENT_WEIGHT)
+                                                break;
+                                        t.step_first = 1;
+                                        t.step_invlb1 = 0;
+                                } else {
+                                        transport_do_ehs(t, material);
+                                        if (c.event & t.step_event) break;
+                                }
+                        } else if (t.step_event & EVENT_MEDIUM) {
+                                medium = new_medium;
+                                if ((medium < 0) || (c.event & EVENT_MEDIUM))
+                                        break;
+                                density = locals_density(medium, &material);
+                                straight = ((c.scattering == MODE_DISABLED) &&
+                                               (scheme <= MODE_MIXED)) ?
+                                    1 :
+                                    0;
+                                const int tmp_scheme =
+                                    scheme > MODE_DISABLED ? scheme : MODE_CSDA;
+                                t.step_X_limit =
+                                    (c.event & EVENT_LIMIT_ENERGY) ?
+                                    cel_grammage(p, tmp_scheme, material,
+                                        c.limit_energy) :
+                                    0.;
+                                t.step_first = 1;
+                                t.step_invlb1 = 0;
+                        }
+
+                        ti = t.time;
+                        wi = t.weight;
+                        Xi = t.grammage;
+                        if (c.energy_loss >= MODE_CSDA) {
+                                const double ki = t.energy;
+                                Xf = cel_grammage(p, scheme, material, ki);
+                                dei = 1. /
+                                    cel_energy_loss(p, scheme, material, ki);
+                        }
+                        transport_limit(t, material, Xi, Xf, &grammage_max);
+                        if (t.step_event) break;
+                }
+
+                if (c.direction == MODE_FORWARD) {
+                        const double kinetic_min =
+                            (c.event & EVENT_LIMIT_ENERGY) ? c.limit_energy :
+                                                             0.;
+                        if (fabs(t.energy - kinetic_min) < FLT_EPS)
+                                t.energy = kinetic_min;
+                } else {
+                        if ((c.event & EVENT_LIMIT_ENERGY) &&
+                            (fabs(t.energy - c.limit_energy) < FLT_EPS))
+                                t.energy = c.limit_energy;
+                }
+
+                *medium_ptr = medium;
+                return t.step_event;
+        }
+
+        /*
+         * pumas_context_transport, src/pumas.c:3114-3251, for one history.
+         * Batch level checks (accuracy, cutoff, decay mode) are done by the
+         * host before the launch. Returns the enum pumas_return code.
+         */
+        PG_FN int transport(Track & t, int * event, int * media) const
+        {
+                *event = EVENT_NONE;
+                media[0] = media[1] = -1;
+                if (t.decayed) {
+                        *event = EVENT_VERTEX_DECAY;
+                        return RETURN_SUCCESS;
+                }
+                {
+                        const double * const u = t.dir;
+                        const double norm2 =
+                            u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+                        if (fabs(norm2 - 1) > FLT_EPS)
+                                return RETURN_DIRECTION_ERROR;
+                }
+
+                int medium;
+                double step_max_medium;
+                int step_max_type = medium_step(t, &medium, &step_max_medium);
+                media[0] = medium;
+                media[1] = -1;
+                if (medium < 0) {
+                        *event = EVENT_MEDIUM;
+                        return RETURN_SUCCESS;
+                } else if ((step_max_medium > 0.) &&
+                    (step_max_type == STEP_CHECK))
+                        step_max_medium += 0.5 * STEP_MIN;
+                int material;
+                const double density = locals_density(medium, &material);
+                const double step_max_locals = 0.;
+                if (density <= 0.) return RETURN_DENSITY_ERROR;
+
+                if (c.decay == MODE_RANDOMISED) {
+                        const double u = t.rng.uniform();
+                        t.lifetime = t.time - p.ctau * log(u);
+                }
+
+                int do_stepping = 1;
+                if ((step_max_medium <= 0.) && (step_max_locals <= 0.) &&
+                    (c.energy_loss <= MODE_CSDA)) {
+                        if ((c.energy_loss == MODE_DISABLED) &&
+                            ((c.event & EVENT_LIMIT) == 0)) {
+                                return RETURN_MISSING_LIMIT;
+                        } else if ((c.scattering == MODE_DISABLED) &&
+                            (c.energy_loss == MODE_CSDA)) {
+                                do_stepping = 0;
+                        }
+                }
+
+                int e;
+                if (do_stepping) {
+                        e = transport_with_stepping(t, &medium, density,
+                            step_max_medium, step_max_type, step_max_locals);
+                } else {
+                        e = transport_with_csda(t, material, density);
+                }
+                *event = e;
+                media[1] = medium;
+                return RETURN_SUCCESS;
+        }
+};
+
+} /* namespace pg */
+
+#endif

@@ -1,0 +1,774 @@
+/*
+ * libpumas_hip: C ABI, table upload and transport kernels for gfx950.
+ *
+ * One work item transports one history at a time. The persistent kernel
+ * keeps a fixed number of wavefronts resident and refills lanes whose
+ * history ended from a global queue, one atomic per wavefront (ballot of
+ * the free lanes and a prefix count to hand out consecutive history ids),
+ * so that uneven history lengths do not leave lanes idle until the longest
+ * history of a wavefront ends.
+ */
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pumas_gpu.h"
+#include "engine_transport.h"
+#include "ptab.h"
+
+using namespace pg;
+
+/* ------------------------------------------------------------------------
+ * Error reporting.
+ * ------------------------------------------------------------------------ */
+static thread_local char g_error[1024] = "";
+
+static enum pumas_return fail(enum pumas_return rc, const char * fmt, ...)
+{
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(g_error, sizeof(g_error), fmt, ap);
+        va_end(ap);
+        return rc;
+}
+
+#define HIP_TRY(expr)                                                          \
+        do {                                                                   \
+                hipError_t e_ = (expr);                                        \
+                if (e_ != hipSuccess)                                          \
+                        return fail(PUMAS_RETURN_INTERNAL_ERROR,               \
+                            "%s: %s (%s:%d)", #expr, hipGetErrorString(e_),    \
+                            __FILE__, __LINE__);                               \
+        } while (0)
+
+extern "C" const char * pumas_gpu_error_message(void) { return g_error; }
+
+extern "C" const char * pumas_gpu_version(void) { return "pumas_hip 0.1.0"; }
+
+extern "C" void pumas_gpu_context_default(struct pumas_gpu_context * context)
+{
+        memset(context, 0x0, sizeof(*context));
+        context->mode.energy_loss = PUMAS_MODE_STRAGGLED;
+        context->mode.decay = PUMAS_MODE_WEIGHTED;
+        context->mode.direction = PUMAS_MODE_FORWARD;
+        context->mode.scattering = PUMAS_MODE_MIXED;
+        context->event = PUMAS_EVENT_NONE;
+        context->accuracy = 1E-02;
+}
+
+/* ------------------------------------------------------------------------
+ * Physics upload.
+ * ------------------------------------------------------------------------ */
+struct pumas_gpu_physics {
+        void * device;          /* one allocation holding every section */
+        size_t size;
+        PhysicsView view;
+        ModelView models;
+        int particle;
+};
+
+namespace {
+
+struct Section {
+        std::string name;
+        uint32_t dtype;
+        uint64_t count;
+        uint64_t offset;
+};
+
+/* Parse a PTAB image; sections index into `image`. */
+enum pumas_return parse_ptab(const uint8_t * image, size_t size,
+    ptab_header * header, std::vector<Section> & sections)
+{
+        if (size < sizeof(ptab_header))
+                return fail(PUMAS_RETURN_FORMAT_ERROR, "PTAB image too small");
+        memcpy(header, image, sizeof(*header));
+        if (memcmp(header->magic, PTAB_MAGIC, 8) != 0)
+                return fail(PUMAS_RETURN_FORMAT_ERROR, "not a PTAB image");
+        if (header->version != PTAB_VERSION)
+                return fail(PUMAS_RETURN_FORMAT_ERROR,
+                    "unsupported PTAB version %u", header->version);
+        const size_t dir_end = sizeof(ptab_header) +
+            (size_t)header->n_entries * sizeof(ptab_entry);
+        if (dir_end > size)
+                return fail(PUMAS_RETURN_INCOMPLETE_FILE, "truncated PTAB");
+        const ptab_entry * e =
+            reinterpret_cast<const ptab_entry *>(image + sizeof(ptab_header));
+        for (uint32_t i = 0; i < header->n_entries; i++) {
+                const size_t width = (e[i].dtype == PTAB_F64) ? 8 : 4;
+                if (e[i].offset + e[i].count * width > size)
+                        return fail(PUMAS_RETURN_INCOMPLETE_FILE,
+                            "truncated PTAB section %s", e[i].name);
+                char name[PTAB_NAME_LEN + 1] = { 0 };
+                memcpy(name, e[i].name, PTAB_NAME_LEN);
+                sections.push_back({ name, e[i].dtype, e[i].count, e[i].offset });
+        }
+        return PUMAS_RETURN_SUCCESS;
+}
+
+const Section * find(const std::vector<Section> & s, const char * name)
+{
+        for (const auto & x : s)
+                if (x.name == name) return &x;
+        return nullptr;
+}
+
+} /* namespace */
+
+extern "C" enum pumas_return pumas_gpu_physics_upload_memory(
+    const void * ptab, size_t size, pumas_gpu_physics ** physics)
+{
+        if (physics == NULL)
+                return fail(PUMAS_RETURN_VALUE_ERROR, "physics is NULL");
+        *physics = NULL;
+        const uint8_t * image = static_cast<const uint8_t *>(ptab);
+        ptab_header h;
+        std::vector<Section> sections;
+        enum pumas_return rc = parse_ptab(image, size, &h, sections);
+        if (rc != PUMAS_RETURN_SUCCESS) return rc;
+
+        const int nM = h.imeta[PTAB_N_MATERIALS];
+        if (h.imeta[PTAB_MAX_COMPONENTS] > MAX_COMPONENTS)
+                return fail(PUMAS_RETURN_VALUE_ERROR,
+                    "materials with more than %d elements are not supported",
+                    MAX_COMPONENTS);
+
+        /* Derived section: first component of each material. */
+        const Section * sin = find(sections, "elements_in");
+        if (sin == nullptr)
+                return fail(PUMAS_RETURN_FORMAT_ERROR, "missing elements_in");
+        std::vector<int32_t> component0(nM);
+        {
+                const int32_t * ein =
+                    reinterpret_cast<const int32_t *>(image + sin->offset);
+                int acc = 0;
+                for (int m = 0; m < nM; m++) {
+                        component0[m] = acc;
+                        acc += ein[m];
+                }
+        }
+
+        /* Device image: the PTAB file as is, then the derived sections. */
+        const size_t extra_offset = (size + 255) / 256 * 256;
+        const size_t total = extra_offset + component0.size() * sizeof(int32_t);
+        void * device = nullptr;
+        HIP_TRY(hipMalloc(&device, total));
+        hipError_t e1 = hipMemcpy(device, image, size, hipMemcpyHostToDevice);
+        hipError_t e2 = hipMemcpy((uint8_t *)device + extra_offset,
+            component0.data(), component0.size() * sizeof(int32_t),
+            hipMemcpyHostToDevice);
+        if ((e1 != hipSuccess) || (e2 != hipSuccess)) {
+                (void)hipFree(device);
+                return fail(PUMAS_RETURN_INTERNAL_ERROR,
+                    "could not copy the tables to the device");
+        }
+
+        pumas_gpu_physics * ph = new pumas_gpu_physics();
+        ph->device = device;
+        ph->size = total;
+        ph->particle = h.imeta[PTAB_PARTICLE];
+        PhysicsView & v = ph->view;
+        v.n_energies = h.imeta[PTAB_N_ENERGIES];
+        v.n_materials = nM;
+        v.n_elements = h.imeta[PTAB_N_ELEMENTS];
+        v.n_components = h.imeta[PTAB_N_COMPONENTS];
+        v.n_table_dcs = h.imeta[PTAB_N_TABLE_DCS];
+        v.max_components = h.imeta[PTAB_MAX_COMPONENTS];
+        v.mass = h.fmeta[PTAB_MASS];
+        v.ctau = h.fmeta[PTAB_CTAU];
+        v.cutoff = h.fmeta[PTAB_CUTOFF];
+
+        const uint8_t * base = static_cast<const uint8_t *>(device);
+        bool missing = false;
+        std::string missing_name;
+        auto dptr = [&](const char * name) -> const void * {
+                const Section * s = find(sections, name);
+                if (s == nullptr) {
+                        missing = true;
+                        missing_name = name;
+                        return nullptr;
+                }
+                return base + s->offset;
+        };
+#define D64(field, name) v.field = (const double *)dptr(name)
+#define F32(field, name) v.field = (const float *)dptr(name)
+#define I32(field, name) v.field = (const int *)dptr(name)
+        D64(K, "K");
+        D64(K_dX, "K_dX");
+        D64(K_dNI_in, "K_dNI_in");
+        D64(K_dNI_el, "K_dNI_el");
+        D64(X, "X");
+        D64(X_dK, "X_dK");
+        D64(X_dT, "X_dT");
+        D64(T, "T");
+        D64(T_dK, "T_dK");
+        D64(dE, "dE");
+        D64(dE_dK, "dE_dK");
+        D64(Omega, "Omega");
+        D64(Omega_dK, "Omega_dK");
+        D64(NI_el, "NI_el");
+        D64(NI_el_dK, "NI_el_dK");
+        D64(NI_in, "NI_in");
+        D64(NI_in_dK, "NI_in_dK");
+        D64(CS, "CS");
+        D64(CS_dK, "CS_dK");
+        D64(CSf, "CSf");
+        D64(CSf_dK, "CSf_dK");
+        D64(CSn, "CSn");
+        D64(CSn_dK, "CSn_dK");
+        D64(Kt, "Kt");
+        D64(a_max, "a_max");
+        D64(b_max, "b_max");
+        D64(Mu0, "Mu0");
+        D64(Mu0_dK, "Mu0_dK");
+        D64(Lb, "Lb");
+        D64(Lb_dK, "Lb_dK");
+        D64(Ms1, "Ms1");
+        D64(Ms1_dK, "Ms1_dK");
+        D64(material_density, "material_density");
+        D64(component_fraction, "component_fraction");
+        D64(element_Z, "element_Z");
+        D64(element_A, "element_A");
+        D64(element_I, "element_I");
+        D64(element_nscr, "element_n_screening");
+        D64(element_amplitude, "element_amplitude");
+        D64(element_exponent, "element_exponent");
+        D64(element_rn, "element_nuclear_radius");
+        D64(element_lrad, "element_radiation_log");
+        F32(DCS, "DCS");
+        F32(DCS_x, "DCS_x");
+        F32(DCS_envelope, "DCS_envelope");
+        I32(elements_in, "elements_in");
+        I32(component_element, "component_element");
+        I32(element_index, "element_index");
+        v.material_component0 = (const int *)(base + extra_offset);
+
+        ModelView & mv = ph->models;
+        mv.z13 = (const double *)dptr("element_z13");
+        mv.zm13 = (const double *)dptr("element_zm13");
+        mv.z23 = (const double *)dptr("element_z23");
+        mv.dn = (const double *)dptr("element_dn");
+        mv.lrad = v.element_lrad;
+        mv.cst = (const double *)dptr("constants");
+        mv.gq9_x = (const double *)dptr("gq9_x");
+        mv.gq9_w = (const double *)dptr("gq9_w");
+        mv.gq12_x = (const double *)dptr("gq12_x");
+        mv.gq12_w = (const double *)dptr("gq12_w");
+#undef D64
+#undef F32
+#undef I32
+        if (missing) {
+                (void)hipFree(device);
+                delete ph;
+                return fail(PUMAS_RETURN_FORMAT_ERROR,
+                    "PTAB section `%s' is missing", missing_name.c_str());
+        }
+        *physics = ph;
+        return PUMAS_RETURN_SUCCESS;
+}
+
+extern "C" enum pumas_return pumas_gpu_physics_upload(
+    const char * path, pumas_gpu_physics ** physics)
+{
+        if (physics != NULL) *physics = NULL;
+        FILE * f = fopen(path, "rb");
+        if (f == NULL)
+                return fail(PUMAS_RETURN_PATH_ERROR, "could not open `%s'", path);
+        fseek(f, 0, SEEK_END);
+        const long size = ftell(f);
+        fseek(f, 0, SEEK_SET);
+        std::vector<uint8_t> image(size > 0 ? size : 0);
+        const size_t n = fread(image.data(), 1, image.size(), f);
+        fclose(f);
+        if (n != image.size())
+                return fail(PUMAS_RETURN_IO_ERROR, "could not read `%s'", path);
+        return pumas_gpu_physics_upload_memory(image.data(), image.size(),
+            physics);
+}
+
+extern "C" void pumas_gpu_physics_destroy(pumas_gpu_physics ** physics)
+{
+        if ((physics == NULL) || (*physics == NULL)) return;
+        (void)hipFree((*physics)->device);
+        delete *physics;
+        *physics = NULL;
+}
+
+extern "C" int pumas_gpu_physics_n_materials(const pumas_gpu_physics * p)
+{
+        return p->view.n_materials;
+}
+
+extern "C" int pumas_gpu_physics_n_energies(const pumas_gpu_physics * p)
+{
+        return p->view.n_energies;
+}
+
+extern "C" double pumas_gpu_physics_cutoff(const pumas_gpu_physics * p)
+{
+        return p->view.cutoff;
+}
+
+/* ------------------------------------------------------------------------
+ * Geometry.
+ * ------------------------------------------------------------------------ */
+struct pumas_gpu_geometry {
+        pg_geometry view;        /* device pointers */
+        std::vector<void *> allocations;
+};
+
+namespace {
+
+template <typename T>
+enum pumas_return upload(pumas_gpu_geometry * g, const T * src, size_t n,
+    const T ** dst)
+{
+        void * d = nullptr;
+        HIP_TRY(hipMalloc(&d, n * sizeof(T) > 0 ? n * sizeof(T) : 1));
+        g->allocations.push_back(d);
+        HIP_TRY(hipMemcpy(d, src, n * sizeof(T), hipMemcpyHostToDevice));
+        *dst = static_cast<const T *>(d);
+        return PUMAS_RETURN_SUCCESS;
+}
+
+} /* namespace */
+
+extern "C" void pumas_gpu_geometry_destroy(pumas_gpu_geometry ** geometry)
+{
+        if ((geometry == NULL) || (*geometry == NULL)) return;
+        for (void * p : (*geometry)->allocations) (void)hipFree(p);
+        delete *geometry;
+        *geometry = NULL;
+}
+
+extern "C" enum pumas_return pumas_gpu_geometry_create(
+    const pumas_gpu_physics * physics,
+    const struct pumas_gpu_geometry_desc * desc, pumas_gpu_geometry ** out)
+{
+        if (out == NULL) return fail(PUMAS_RETURN_VALUE_ERROR, "NULL output");
+        *out = NULL;
+        if ((physics == NULL) || (desc == NULL))
+                return fail(PUMAS_RETURN_VALUE_ERROR, "NULL argument");
+        if ((desc->n_media <= 0) || (desc->media == NULL))
+                return fail(PUMAS_RETURN_MEDIUM_ERROR, "no media");
+        for (int i = 0; i < desc->n_media; i++) {
+                const int m = desc->media[i].material;
+                if ((m < 0) || (m >= physics->view.n_materials))
+                        return fail(PUMAS_RETURN_INDEX_ERROR,
+                            "invalid material index [%d] for medium %d", m, i);
+        }
+
+        pumas_gpu_geometry * g = new pumas_gpu_geometry();
+        pg_geometry & v = g->view;
+        memset(&v, 0x0, sizeof(v));
+        v.kind = desc->kind;
+        v.step_type = desc->step_type;
+        v.n_media = desc->n_media;
+        std::vector<int> material(desc->n_media);
+        std::vector<double> density(desc->n_media);
+        for (int i = 0; i < desc->n_media; i++) {
+                material[i] = desc->media[i].material;
+                density[i] = desc->media[i].density;
+        }
+        enum pumas_return rc;
+#define UP(src, n, dst)                                                        \
+        do {                                                                   \
+                rc = upload(g, src, n, dst);                                   \
+                if (rc != PUMAS_RETURN_SUCCESS) {                              \
+                        pumas_gpu_geometry_destroy(&g);                        \
+                        return rc;                                             \
+                }                                                              \
+        } while (0)
+        UP(material.data(), material.size(), &v.material);
+        UP(density.data(), density.size(), &v.density);
+
+        switch (desc->kind) {
+        case PUMAS_GPU_GEOMETRY_INFINITE:
+                break;
+        case PUMAS_GPU_GEOMETRY_SLABS:
+                if ((desc->n_layers <= 0) || (desc->z == NULL) ||
+                    (desc->n_media < desc->n_layers)) {
+                        pumas_gpu_geometry_destroy(&g);
+                        return fail(PUMAS_RETURN_VALUE_ERROR,
+                            "slabs need n_layers + 1 boundaries and media");
+                }
+                for (int i = 0; i < desc->n_layers; i++)
+                        if (!(desc->z[i] < desc->z[i + 1])) {
+                                pumas_gpu_geometry_destroy(&g);
+                                return fail(PUMAS_RETURN_VALUE_ERROR,
+                                    "slab boundaries must increase");
+                        }
+                v.n_layers = desc->n_layers;
+                UP(desc->z, (size_t)desc->n_layers + 1, &v.z);
+                break;
+        case PUMAS_GPU_GEOMETRY_TERRAIN: {
+                if ((desc->nx < 2) || (desc->ny < 2) || (desc->height == NULL) ||
+                    (desc->density == NULL) || (desc->n_media < 2) ||
+                    (desc->dx <= 0.) || (desc->dy <= 0.)) {
+                        pumas_gpu_geometry_destroy(&g);
+                        return fail(PUMAS_RETURN_VALUE_ERROR,
+                            "invalid terrain description");
+                }
+                v.nx = desc->nx;
+                v.ny = desc->ny;
+                v.x0 = desc->x0;
+                v.y0 = desc->y0;
+                v.z0 = desc->z0;
+                v.dx = desc->dx;
+                v.dy = desc->dy;
+                v.z_top = desc->z_top;
+                UP(desc->height, (size_t)desc->nx * desc->ny, &v.height);
+                UP(desc->density, (size_t)(desc->nx - 1) * (desc->ny - 1),
+                    &v.cell_density);
+                break;
+        }
+        case PUMAS_GPU_GEOMETRY_VOXELS: {
+                if ((desc->nx < 1) || (desc->ny < 1) || (desc->nz < 1) ||
+                    (desc->label == NULL) || (desc->density == NULL) ||
+                    (desc->dx <= 0.) || (desc->dy <= 0.) || (desc->dz <= 0.)) {
+                        pumas_gpu_geometry_destroy(&g);
+                        return fail(PUMAS_RETURN_VALUE_ERROR,
+                            "invalid voxel description");
+                }
+                const size_t nv = (size_t)desc->nx * desc->ny * desc->nz;
+                if (nv >= (size_t)0x7fffffff) {
+                        pumas_gpu_geometry_destroy(&g);
+                        return fail(PUMAS_RETURN_VALUE_ERROR,
+                            "too many voxels");
+                }
+                for (size_t i = 0; i < nv; i++)
+                        if (desc->label[i] >= desc->n_media) {
+                                pumas_gpu_geometry_destroy(&g);
+                                return fail(PUMAS_RETURN_INDEX_ERROR,
+                                    "voxel %zu has an invalid label", i);
+                        }
+                v.nx = desc->nx;
+                v.ny = desc->ny;
+                v.nz = desc->nz;
+                v.x0 = desc->x0;
+                v.y0 = desc->y0;
+                v.z0 = desc->z0;
+                v.dx = desc->dx;
+                v.dy = desc->dy;
+                v.dz = desc->dz;
+                UP(desc->label, nv, &v.label);
+                UP(desc->density, nv, &v.cell_density);
+                break;
+        }
+        default:
+                pumas_gpu_geometry_destroy(&g);
+                return fail(PUMAS_RETURN_VALUE_ERROR, "unknown geometry kind");
+        }
+#undef UP
+        *out = g;
+        return PUMAS_RETURN_SUCCESS;
+}
+
+/* ------------------------------------------------------------------------
+ * Transport kernel.
+ * ------------------------------------------------------------------------ */
+struct KernelArgs {
+        PhysicsView p;
+        ModelView mv;
+        Config c;
+        pg_geometry g;
+        struct pumas_state * states;
+        uint64_t n;
+        uint64_t seed;
+        uint64_t first;
+        enum pumas_event * events;
+        int32_t * media;
+        uint32_t * steps;
+        uint32_t * draws;
+        int32_t * rc;
+        int max_steps;
+        unsigned long long * counters; /* [0] queue, [1] steps, [2] error */
+};
+
+__device__ static void run_history(const KernelArgs & a, uint64_t i,
+    unsigned long long & steps_acc)
+{
+        const Engine eng = { a.p, a.mv, a.c, a.g };
+        struct pumas_state * s = a.states + i;
+        Track t;
+        t.charge = s->charge;
+        t.energy = s->energy;
+        t.distance = s->distance;
+        t.grammage = s->grammage;
+        t.time = s->time;
+        t.weight = s->weight;
+        t.pos[0] = s->position[0];
+        t.pos[1] = s->position[1];
+        t.pos[2] = s->position[2];
+        t.dir[0] = s->direction[0];
+        t.dir[1] = s->direction[1];
+        t.dir[2] = s->direction[2];
+        t.decayed = s->decayed;
+        t.step_first = 0;
+        t.step_event = t.step_foreseen = 0;
+        t.step_X_limit = t.step_invlb1 = t.lifetime = 0.;
+        t.randn_done = 0;
+        t.randn_next = 0.;
+        t.rng.init(a.seed, a.first + i, STREAM_TRANSPORT, 0);
+        t.steps = 0;
+        t.max_steps = a.max_steps;
+
+        int event, media[2];
+        const int rc = eng.transport(t, &event, media);
+
+        s->charge = t.charge;
+        s->energy = t.energy;
+        s->distance = t.distance;
+        s->grammage = t.grammage;
+        s->time = t.time;
+        s->weight = t.weight;
+        s->position[0] = t.pos[0];
+        s->position[1] = t.pos[1];
+        s->position[2] = t.pos[2];
+        s->direction[0] = t.dir[0];
+        s->direction[1] = t.dir[1];
+        s->direction[2] = t.dir[2];
+        s->decayed = t.decayed;
+        if (a.events) a.events[i] = (enum pumas_event)event;
+        if (a.media) {
+                a.media[2 * i] = media[0];
+                a.media[2 * i + 1] = media[1];
+        }
+        if (a.steps) a.steps[i] = t.steps;
+        if (a.draws) a.draws[i] = t.rng.count;
+        if (a.rc) a.rc[i] = rc;
+        if (rc != RETURN_SUCCESS)
+                atomicCAS(a.counters + 2, 0ull, (unsigned long long)rc);
+        steps_acc += t.steps;
+}
+
+/*
+ * Persistent transport kernel. Each wavefront loops: lanes without a
+ * history take the next ids from the global queue with a single atomic per
+ * wavefront, then every lane transports its history to completion.
+ */
+__global__ void __launch_bounds__(256) transport_persistent(KernelArgs a)
+{
+        const int lane = threadIdx.x & 63;
+        unsigned long long steps_acc = 0;
+        for (;;) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(a.counters, 64ull);
+                base = __shfl(base, 0, 64);
+                if (base >= a.n) break;
+                const uint64_t i = base + lane;
+                if (i < a.n) run_history(a, i, steps_acc);
+        }
+        /* Wavefront reduction of the step count, one atomic per wave. */
+        for (int off = 32; off > 0; off >>= 1)
+                steps_acc += __shfl_down(steps_acc, off, 64);
+        if (lane == 0) atomicAdd(a.counters + 1, steps_acc);
+}
+
+/* Clears the queue and tally counters (a kernel, not a memset node, so
+ * that the launch sequence can be captured in a graph). */
+__global__ void zero_counters(unsigned long long * c, int n)
+{
+        if ((int)threadIdx.x < n) c[threadIdx.x] = 0ull;
+}
+
+/* ------------------------------------------------------------------------
+ * C ABI transport.
+ * ------------------------------------------------------------------------ */
+namespace {
+
+enum pumas_return check_context(const pumas_gpu_physics * physics,
+    const struct pumas_gpu_context * c)
+{
+        if ((c->accuracy <= 0) || (c->accuracy > 1))
+                return fail(PUMAS_RETURN_ACCURACY_ERROR,
+                    "bad accuracy value (expected a value in ]0,1], got %g)",
+                    c->accuracy);
+        if ((c->mode.direction == PUMAS_MODE_BACKWARD) &&
+            (c->mode.energy_loss > PUMAS_MODE_CSDA) &&
+            (physics->view.cutoff < 1E-02))
+                return fail(PUMAS_RETURN_ACCURACY_ERROR,
+                    "bad cutoff value for backward transport (expected a "
+                    "value greater than or equal to 0.01, got %g)",
+                    physics->view.cutoff);
+        if ((physics->particle != 0) &&
+            (c->mode.direction == PUMAS_MODE_FORWARD) &&
+            (c->mode.decay == PUMAS_MODE_WEIGHTED))
+                return fail(PUMAS_RETURN_DECAY_ERROR,
+                    "`PUMAS_MODE_WEIGHTED' mode is not valid for forward taus");
+        return PUMAS_RETURN_SUCCESS;
+}
+
+} /* namespace */
+
+extern "C" enum pumas_return pumas_gpu_transport(
+    const pumas_gpu_physics * physics, const struct pumas_gpu_context * context,
+    const pumas_gpu_geometry * geometry, struct pumas_state * states, size_t n,
+    uint64_t seed, uint64_t first_history,
+    const struct pumas_gpu_outputs * outputs,
+    const struct pumas_gpu_options * options, struct pumas_gpu_stats * stats,
+    void * stream_)
+{
+        g_error[0] = '\0';
+        if ((physics == NULL) || (context == NULL) || (geometry == NULL))
+                return fail(PUMAS_RETURN_VALUE_ERROR, "NULL argument");
+        if ((states == NULL) && (n > 0))
+                return fail(PUMAS_RETURN_VALUE_ERROR, "no states (null)");
+        enum pumas_return rc = check_context(physics, context);
+        if (rc != PUMAS_RETURN_SUCCESS) return rc;
+        if (options && options->precision != PUMAS_GPU_PRECISION_FP64)
+                return fail(PUMAS_RETURN_VALUE_ERROR,
+                    "precision mode %d is not available", options->precision);
+        hipStream_t stream = static_cast<hipStream_t>(stream_);
+
+        KernelArgs a;
+        memset(&a, 0x0, sizeof(a));
+        a.p = physics->view;
+        a.mv = physics->models;
+        a.c.energy_loss = context->mode.energy_loss;
+        a.c.decay = context->mode.decay;
+        a.c.direction = context->mode.direction;
+        a.c.scattering = context->mode.scattering;
+        a.c.event = context->event;
+        a.c.limit_energy = context->limit.energy;
+        a.c.limit_distance = context->limit.distance;
+        a.c.limit_grammage = context->limit.grammage;
+        a.c.limit_time = context->limit.time;
+        a.c.accuracy = context->accuracy;
+        a.g = geometry->view;
+        a.states = states;
+        a.n = n;
+        a.seed = seed;
+        a.first = first_history;
+        if (outputs) {
+                a.events = outputs->events;
+                a.media = outputs->media;
+                a.steps = outputs->steps;
+                a.draws = outputs->draws;
+                a.rc = outputs->rc;
+        }
+        a.max_steps = (options && options->max_steps > 0) ? options->max_steps :
+                                                            10000000;
+
+        int device;
+        HIP_TRY(hipGetDevice(&device));
+        int n_cu = 0;
+        HIP_TRY(hipDeviceGetAttribute(
+            &n_cu, hipDeviceAttributeMultiprocessorCount, device));
+        const int block = (options && options->block_size > 0) ?
+            options->block_size :
+            256;
+        if ((block % 64) || (block > 256))
+                return fail(PUMAS_RETURN_VALUE_ERROR,
+                    "block size must be a multiple of 64, at most 256");
+        const int per_cu = (options && options->blocks_per_cu > 0) ?
+            options->blocks_per_cu :
+            4;
+        size_t grid = (size_t)n_cu * per_cu;
+        const size_t needed = (n + block - 1) / block;
+        if (needed < grid) grid = needed;
+        if (grid == 0) grid = 1;
+
+        unsigned long long * counters = nullptr;
+        HIP_TRY(hipMallocAsync((void **)&counters, 4 * sizeof(unsigned long long),
+            stream));
+        hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, stream,
+            counters, 4);
+        HIP_TRY(hipGetLastError());
+        a.counters = counters;
+
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (stats) {
+                HIP_TRY(hipEventCreate(&e0));
+                HIP_TRY(hipEventCreate(&e1));
+                HIP_TRY(hipEventRecord(e0, stream));
+        }
+        if (n > 0) {
+                hipLaunchKernelGGL(transport_persistent, dim3((unsigned)grid),
+                    dim3(block), 0, stream, a);
+                HIP_TRY(hipGetLastError());
+        }
+        unsigned long long host_counters[4] = { 0, 0, 0, 0 };
+        if (stats) HIP_TRY(hipEventRecord(e1, stream));
+        HIP_TRY(hipMemcpyAsync(host_counters, counters, sizeof(host_counters),
+            hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipFreeAsync(counters, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        if (stats) {
+                float ms = 0.f;
+                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                stats->steps = host_counters[1];
+                stats->histories = n;
+                stats->kernel_ms = ms;
+        }
+        if (host_counters[2] != 0)
+                return fail((enum pumas_return)host_counters[2],
+                    "a history raised error code %llu", host_counters[2]);
+        return PUMAS_RETURN_SUCCESS;
+}
+
+extern "C" enum pumas_return pumas_gpu_transport_host(
+    const pumas_gpu_physics * physics, const struct pumas_gpu_context * context,
+    const pumas_gpu_geometry * geometry, struct pumas_state * states, size_t n,
+    uint64_t seed, uint64_t first_history,
+    const struct pumas_gpu_outputs * outputs,
+    const struct pumas_gpu_options * options, struct pumas_gpu_stats * stats)
+{
+        struct pumas_state * d_states = nullptr;
+        struct pumas_gpu_outputs d_out;
+        memset(&d_out, 0x0, sizeof(d_out));
+        const size_t ns = n > 0 ? n : 1;
+        HIP_TRY(hipMalloc(&d_states, ns * sizeof(*states)));
+        HIP_TRY(hipMemcpy(d_states, states, n * sizeof(*states),
+            hipMemcpyHostToDevice));
+        if (outputs) {
+                if (outputs->events)
+                        HIP_TRY(hipMalloc(&d_out.events, ns * sizeof(int)));
+                if (outputs->media)
+                        HIP_TRY(hipMalloc(&d_out.media, 2 * ns * sizeof(int32_t)));
+                if (outputs->steps)
+                        HIP_TRY(hipMalloc(&d_out.steps, ns * sizeof(uint32_t)));
+                if (outputs->draws)
+                        HIP_TRY(hipMalloc(&d_out.draws, ns * sizeof(uint32_t)));
+                if (outputs->rc)
+                        HIP_TRY(hipMalloc(&d_out.rc, ns * sizeof(int32_t)));
+        }
+        const enum pumas_return rc = pumas_gpu_transport(physics, context,
+            geometry, d_states, n, seed, first_history, &d_out, options, stats,
+            nullptr);
+        const std::string message = g_error;
+        HIP_TRY(hipMemcpy(states, d_states, n * sizeof(*states),
+            hipMemcpyDeviceToHost));
+        if (outputs) {
+                if (outputs->events)
+                        HIP_TRY(hipMemcpy(outputs->events, d_out.events,
+                            n * sizeof(int), hipMemcpyDeviceToHost));
+                if (outputs->media)
+                        HIP_TRY(hipMemcpy(outputs->media, d_out.media,
+                            2 * n * sizeof(int32_t), hipMemcpyDeviceToHost));
+                if (outputs->steps)
+                        HIP_TRY(hipMemcpy(outputs->steps, d_out.steps,
+                            n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+                if (outputs->draws)
+                        HIP_TRY(hipMemcpy(outputs->draws, d_out.draws,
+                            n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+                if (outputs->rc)
+                        HIP_TRY(hipMemcpy(outputs->rc, d_out.rc,
+                            n * sizeof(int32_t), hipMemcpyDeviceToHost));
+        }
+        (void)hipFree(d_states);
+        (void)hipFree(d_out.events);
+        (void)hipFree(d_out.media);
+        (void)hipFree(d_out.steps);
+        (void)hipFree(d_out.draws);
+        (void)hipFree(d_out.rc);
+        snprintf(g_error, sizeof(g_error), "%s", message.c_str());
+        return rc;
+}
