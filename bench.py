@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""Benchmark: batched muon transport on one MI355X.
+
+A step of this benchmark is one pass of `pumas_gpu_transport` over the whole
+batch of the forward slab workload: 10^7 muons with kinetic energy
+log-uniform in [10, 10^4] GeV and cos(zenith) uniform in [0.5, 1], through
+100 m of air, 500 m of standard rock and 200 m of water, straggled energy
+loss (class II discrete losses above a 5% cutoff) and mixed multiple
+scattering, FP64. Each pass uses fresh random streams (history ids offset by
+the pass index) on the same initial states, which are resident in HBM
+before the timed region.
+
+The reported value is muon-steps per second (one step = one Monte Carlo
+step of pumas_context_transport, i.e. one geometry step proposal); events
+per second are reported beside it. The CPU baseline is the reference library
+(compiled from its source by oracle/Makefile) with one context per thread on
+the host cores, over a bounded sample of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+# Algorithmic HBM bytes per Monte Carlo step of the FP64 stepping record:
+# struct pumas_state (104 B) plus the per-history stepping carry (96 B),
+# read and written once per step.
+BYTES_PER_STEP_FP64 = 2 * (104 + 96)
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=10_000_000,
+                    help="histories per pass (the workload has 10^7)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="target duration of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--block-size", type=int, default=0)
+    ap.add_argument("--blocks-per-cu", type=int, default=0)
+    return ap.parse_args()
+
+
+def cpu_baseline(states, seconds):
+    """Reference CPU path, one context per thread, on a bounded sample."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import oracle as orc
+    from pumas_amd import workloads as W
+    if not orc.available():
+        return None
+    threads = min(16, len(os.sched_getaffinity(0)))
+    phys = orc.RefPhysics("standard_c005")
+    geom = orc.RefGeometry(W.slabs_geometry())
+    ctx = W.slabs_forward_context()
+    # Calibrate on a small sample, then size the sample to `seconds`.
+    n = 256 * threads
+    r = orc.transport(phys, ctx, geom, states[:n], seed=0, threads=threads)
+    rate = n / max(r.seconds, 1e-3)
+    n = int(min(len(states), max(n, rate * seconds)))
+    r = orc.transport(phys, ctx, geom, states[:n], seed=0, threads=threads)
+    phys.close()
+    return {
+        "value": r.total_steps / r.seconds,
+        "unit": "muon-steps/s",
+        "events_per_s": n / r.seconds,
+        "cores": threads,
+        "kind": "reference",
+        "sample": f"{n} histories of the same workload ({r.total_steps} "
+                  f"steps, {r.seconds:.1f} s)",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU "
+                          "sharding is described in DESIGN.md and not "
+                          "launched"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import pumas_amd as pa
+    from pumas_amd import workloads as W
+
+    n = args.n
+    log(f"bench: generating {n} initial states")
+    states = W.slabs_forward_states(n, seed=0)
+    phys = pa.Physics(pa.physics_path("standard_c005"))
+    geom = pa.Geometry(phys, W.slabs_geometry())
+    ctx = W.slabs_forward_context()
+
+    dev = torch.device("cuda:0")
+    pristine = torch.from_numpy(states.view(np.uint8).reshape(-1)).to(dev)
+    work = torch.empty_like(pristine)
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def one_pass(k):
+        work.copy_(pristine)
+        return pa.transport_device(
+            phys, ctx, geom, work.data_ptr(), n, seed=0,
+            first_history=k * n, stream=stream,
+            block_size=args.block_size, blocks_per_cu=args.blocks_per_cu)
+
+    for k in range(args.warmup):
+        s, ms = one_pass(1000 + k)
+        log(f"bench: warmup pass {k}: {ms:.1f} ms, {s} steps")
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    total_steps, kernel_ms = 0, []
+    for k in range(args.steps):
+        s, ms = one_pass(k)
+        total_steps += s
+        kernel_ms.append(ms)
+        log(f"bench: pass {k}: {ms:.1f} ms, {s} steps")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+
+    steps_per_s = total_steps / elapsed
+    events_per_s = args.steps * n / elapsed
+    mean_ms = sum(kernel_ms) / len(kernel_ms)
+    steps_per_launch = total_steps / args.steps
+    achieved = steps_per_launch * BYTES_PER_STEP_FP64 / (mean_ms * 1e-3) / 1e9
+
+    traffic = None
+    tpath = os.path.join(HERE, "profiles", "transport_persistent_slabs_traffic.json")
+    if os.path.exists(tpath) and n == 10_000_000:
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        log("bench: CPU baseline (reference, all host cores)")
+        cpu = cpu_baseline(states, args.cpu_seconds)
+
+    line = {
+        "metric": "muon-steps/s & events/s at <=0.1% flux bias vs FP64; "
+                  "p50/p99 request latency",
+        "value": steps_per_s,
+        "unit": "muon-steps/s",
+        "events_per_s": events_per_s,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "forward straggled + mixed scattering, 3 slabs "
+                        "(100 m air / 500 m standard rock / 200 m water), "
+                        "K0 log-uniform [10, 1e4] GeV, cos(zenith) in "
+                        "[0.5, 1], FP64",
+            "histories": n,
+            "precision": "fp64",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "transport_persistent",
+            "kernel_ms": mean_ms,
+            "bytes_per_step": BYTES_PER_STEP_FP64,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -18,6 +18,7 @@ from pumas_amd._abi import STATE_DTYPE
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 LIBRARY = os.path.join(HERE, "_ref", "libpumas_ref.so")
+LIBRARY_ALIGNED = os.path.join(HERE, "_ref", "libpumas_ref_aligned.so")
 MDF = {
     "standard_c005": (os.path.join(REPO, "pumas_amd", "export",
                                    "materials_standard.xml"), 0.05),
@@ -25,20 +26,22 @@ MDF = {
                                "materials_iron.xml"), 0.01),
 }
 
-_lib = None
+_libs = {}
 
 
-def available():
-    return os.path.exists(LIBRARY)
+def available(aligned=False):
+    return os.path.exists(LIBRARY_ALIGNED if aligned else LIBRARY)
 
 
-def library():
-    global _lib
-    if _lib is None:
-        if not available():
+def library(aligned=False):
+    """The oracle library; `aligned` selects the build whose float logf /
+    expf are rounded double precision evaluations."""
+    if aligned not in _libs:
+        path = LIBRARY_ALIGNED if aligned else LIBRARY
+        if not os.path.exists(path):
             raise FileNotFoundError(
-                f"{LIBRARY} is missing; build it with `make -C oracle`")
-        lib = ctypes.CDLL(LIBRARY)
+                f"{path} is missing; build it with `make -C oracle`")
+        lib = ctypes.CDLL(path)
         vp, P = ctypes.c_void_p, ctypes.POINTER
         lib.ref_physics_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p,
                                            ctypes.c_double, P(vp)]
@@ -74,41 +77,43 @@ def library():
         lib.ref_dcs_exact.restype = ctypes.c_double
         lib.ref_dilog.argtypes = [ctypes.c_double]
         lib.ref_dilog.restype = ctypes.c_double
-        _lib = lib
-    return _lib
+        _libs[aligned] = lib
+    return _libs[aligned]
 
 
 class RefPhysics:
     """Reference physics created by pumas_physics_create from the MDF the
     PTAB file of the same name was exported from."""
 
-    def __init__(self, name):
+    def __init__(self, name, aligned=False):
         mdf, cutoff = MDF[name]
         self.name = name
+        self.aligned = aligned
+        self.lib = library(aligned)
         self._dedx = tempfile.mkdtemp(prefix="pumas_dedx_")
         self.handle = ctypes.c_void_p()
-        rc = library().ref_physics_create(mdf.encode(), self._dedx.encode(),
-                                          cutoff, ctypes.byref(self.handle))
+        rc = self.lib.ref_physics_create(mdf.encode(), self._dedx.encode(),
+                                         cutoff, ctypes.byref(self.handle))
         if rc != 0:
             raise RuntimeError(f"pumas_physics_create failed ({rc})")
 
     def export(self, path):
-        rc = library().ref_physics_export(self.handle, str(path).encode())
+        rc = self.lib.ref_physics_export(self.handle, str(path).encode())
         if rc != 0:
             raise RuntimeError("export failed")
 
     def cel(self, which, scheme, material, x):
-        return library().ref_cel(self.handle, which, scheme, material, x)
+        return self.lib.ref_cel(self.handle, which, scheme, material, x)
 
     def dcs_evaluate(self, process, element, K, q):
-        return library().ref_dcs_evaluate(self.handle, process, element, K, q)
+        return self.lib.ref_dcs_evaluate(self.handle, process, element, K, q)
 
     def dcs_exact(self, process, element, K, q):
-        return library().ref_dcs_exact(self.handle, process, element, K, q)
+        return self.lib.ref_dcs_exact(self.handle, process, element, K, q)
 
     def close(self):
         if self.handle:
-            library().ref_physics_destroy(self.handle)
+            self.lib.ref_physics_destroy(self.handle)
             self.handle = ctypes.c_void_p()
         shutil.rmtree(self._dedx, ignore_errors=True)
 
@@ -120,14 +125,15 @@ class RefPhysics:
 
 
 class RefGeometry:
-    def __init__(self, description):
+    def __init__(self, description, aligned=False):
         self.description = description
+        self.lib = library(aligned)
         self._desc = description.to_c()
-        self.handle = library().ref_geometry_create(ctypes.byref(self._desc))
+        self.handle = self.lib.ref_geometry_create(ctypes.byref(self._desc))
 
     def __del__(self):
         try:
-            library().ref_geometry_destroy(self.handle)
+            self.lib.ref_geometry_destroy(self.handle)
         except Exception:
             pass
 
@@ -150,7 +156,9 @@ def transport(physics, context, geometry, states, seed=0, first=0,
     r.rc = np.zeros(n, np.int32)
     total = ctypes.c_uint64()
     ctx = context.to_c()
-    r.seconds = library().ref_transport(
+    if geometry.lib is not physics.lib:
+        raise ValueError("physics and geometry come from different oracles")
+    r.seconds = physics.lib.ref_transport(
         physics.handle, ctypes.byref(ctx), geometry.handle,
         states.ctypes.data, n, seed, first, r.events.ctypes.data,
         r.media.ctypes.data, r.steps.ctypes.data, r.draws.ctypes.data,

@@ -125,6 +125,10 @@ class Context:
     limit_time: float = 0.0
     accuracy: float = 1e-2
 
+    def replace(self, **changes):
+        """A copy with some fields changed."""
+        return dataclasses.replace(self, **changes)
+
     def to_c(self):
         c = _abi.GpuContext()
         c.mode.energy_loss = int(self.energy_loss)
@@ -273,12 +277,15 @@ def _options(precision, block_size, blocks_per_cu, max_steps):
 
 def transport(physics, context, geometry, states, seed=0, first_history=0,
               outputs=True, precision=Precision.FP64, block_size=0,
-              blocks_per_cu=0, max_steps=0):
+              blocks_per_cu=0, max_steps=0, raise_errors=True):
     """Transport host `states` (a STATE_DTYPE array, updated in place).
 
     Batch counterpart of pumas_context_transport: history i draws from the
     Philox stream keyed by `seed` at history id `first_history + i`.
     Returns a TransportResult with the per-history outputs when `outputs`.
+    A history that raises an error leaves the others unaffected; the first
+    error is raised as PumasError, or stored in `result.error` when
+    `raise_errors` is false.
     """
     lib = library()
     states = np.ascontiguousarray(states, dtype=STATE_DTYPE)
@@ -299,10 +306,13 @@ def transport(physics, context, geometry, states, seed=0, first_history=0,
     stats = _abi.GpuStats()
     opts = _options(precision, block_size, blocks_per_cu, max_steps)
     ctx = context.to_c()
-    _check(lib.pumas_gpu_transport_host(
+    rc = lib.pumas_gpu_transport_host(
         physics.handle, ctypes.byref(ctx), geometry.handle,
         states.ctypes.data, n, seed, first_history, ctypes.byref(out),
-        ctypes.byref(opts), ctypes.byref(stats)))
+        ctypes.byref(opts), ctypes.byref(stats))
+    res.error = Return(rc)
+    if raise_errors:
+        _check(rc)
     res.total_steps = int(stats.steps)
     res.kernel_ms = float(stats.kernel_ms)
     return res

@@ -120,35 +120,47 @@ PG_FN void dcs_pair_production_range(const ModelView & mv, int iel, double m,
 }
 
 /*
- * Real dilogarithm Li2(x) for 0 < x < 1 (the range used by the pair
- * production DDCS). Bernoulli series in u = -ln(1 - x) for x <= 1/2
- * ('t Hooft & Veltman, Nucl. Phys. B153 (1979) 365), with the reflection
- * Li2(x) = pi^2/6 - ln(x) ln(1 - x) - Li2(1 - x) above. The reference uses
- * the CERNLIB RDILOG Chebyshev expansion (src/pumas.c:12609-12686); both
- * agree to double precision rounding.
+ * Real dilogarithm by the Chebyshev expansion of CERNLIB RDILOG (C332), the
+ * method of math_dilog, src/pumas.c:12609-12686, with the same argument
+ * transformations and coefficients so that the pair production DDCS rounds
+ * identically (CERN Program Library, http://cds.cern.ch/record/2050865).
+ * Only the branches reached by 0 < x < 1 are kept.
  */
-PG_FN double dilog01(double x)
+PG_FN double dilog(double x)
 {
-        /* B_2n / (2n + 1)! for n = 1..11. */
-        const double b[11] = { 2.7777777777777778e-02, -2.7777777777777778e-04,
-                4.7241118669690098e-06, -9.1857730746619635e-08,
-                1.8978869988971001e-09, -4.0647616451442255e-11,
-                8.9216910204564526e-13, -1.9939295860721075e-14,
-                4.5189800296199182e-16, -1.0356517612181247e-17,
-                2.3952186210261868e-19 };
-        double sign = 1., add = 0.;
-        double y = x;
-        if (x > 0.5) {
-                y = 1. - x;
-                sign = -1.;
-                add = PI * PI / 6. - log(x) * log(y);
+        const double C[20] = { 0.42996693560813697, 0.40975987533077105,
+                -0.01858843665014592, 0.00145751084062268,
+                -0.00014304184442340, 0.00001588415541880,
+                -0.00000190784959387, 0.00000024195180854,
+                -0.00000003193341274, 0.00000000434545063,
+                -0.00000000060578480, 0.00000000008612098,
+                -0.00000000001244332, 0.00000000000182256,
+                -0.00000000000027007, 0.00000000000004042,
+                -0.00000000000000610, 0.00000000000000093,
+                -0.00000000000000014, 0.00000000000000002 };
+        const double PI6 = PI * PI / 6.;
+        const double T = -x;
+        double A, S, Y;
+        if (T <= -0.5) {
+                Y = -(1. + T) / T;
+                S = 1.;
+                A = log(-T);
+                A = -PI6 + A * (-0.5 * A + log(1. + T));
+        } else {
+                Y = -T / (1. + T);
+                S = -1.;
+                const double l = log(1 + T);
+                A = 0.5 * (l * l);
         }
-        const double u = -log(1. - y);
-        const double u2 = u * u;
-        double s = 0.;
-        for (int i = 10; i >= 0; i--) s = s * u2 + b[i];
-        const double li2 = u - 0.25 * u2 + u * u2 * s;
-        return add + sign * li2;
+        const double H = Y + Y - 1;
+        const double ALFA = H + H;
+        double B0 = 0., B1 = 0., B2 = 0.;
+        for (int i = 19; i >= 0; i--) {
+                B0 = C[i] + ALFA * B1 - B2;
+                B2 = B1;
+                B1 = B0;
+        }
+        return -(S * (B0 - H * B2) + A);
 }
 
 /*
@@ -215,7 +227,7 @@ PG_FN double dcs_pair_production_d2_SSR(const ModelView & mv, int iel,
         const double Ce1 = Be - Ce2;
 
         const double De = ((2. + rho2) * (1. + beta) +
-            xi * (3. + rho2)) * dilog01(1. / (1. + xi)) -
+            xi * (3. + rho2)) * dilog(1. / (1. + xi)) -
             (2. + rho2) * xi * log(1. + 1. / xi) -
             (xi + rho2 + beta) / (1. + xi);
 
@@ -266,7 +278,7 @@ PG_FN double dcs_pair_production_d2_SSR(const ModelView & mv, int iel,
 
         const double Dm = ((1. + rho2) * (1. + (3. * beta) / 2.) -
             1. / xi * (1. + 2. * beta) * (1. - rho2)) *
-            dilog01(xi / (1. + xi)) + (1. + (3. * beta) / 2.) *
+            dilog(xi / (1. + xi)) + (1. + (3. * beta) / 2.) *
             (1. - rho2) / xi * log(1. + xi) + (1. - rho2 - beta / 2. *
             (1. + rho2) + (1. - rho2) / (2. * xi) * beta) * xi / (1. + xi);
 

@@ -973,12 +973,19 @@ struct Engine {
                                 zeta = t.rng.uniform() * stot;
                                 if ((zeta > 0.) && (zeta < stot)) break;
                         }
+                        /* The reference keeps the energy transfer and the
+                         * weight in one union (struct del_info,
+                         * src/pumas.c:612-626); once the weight is set to
+                         * the summed DCS, the selection pass below reads
+                         * that value as the energy transfer
+                         * (src/pumas.c:6229-6243). Kept for parity. */
+                        const double Q2 = stot;
                         double s = 0.;
                         for (int ic = ic0; ic < ic0 + nc; ic++)
                                 for (int ip = 0; ip < N_DEL_PROCESSES; ip++) {
                                         const int iel = p.component_element[ic];
                                         const double si = dcs_evaluate(p, mv,
-                                            ip, iel, t.energy, Q) *
+                                            ip, iel, t.energy, Q2) *
                                             p.component_fraction[ic];
                                         s += si;
                                         if (!(zeta > s)) {

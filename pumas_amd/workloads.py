@@ -152,3 +152,33 @@ def slabs_forward_context(energy_loss=Mode.STRAGGLED):
 
 def slabs_geometry(step_type=Step.RAW):
     return GeometryDescription.slabs(SLAB_Z, SLAB_MEDIA, step_type=step_type)
+
+
+def slabs_backward_states(n, seed=0, first=0, kmin=1e-1, kmax=1e3,
+                          cmin=0.5, cmax=1.0):
+    """Final states at a detector at the bottom of the stack: K log-uniform
+    in [kmin, kmax] GeV, momentum going down with cos(zenith) uniform in
+    [cmin, cmax], weight 1/pdf, charge -1."""
+    ids = _ids(n, first)
+    u0 = uniform(seed, 0, ids, 0)
+    u1 = uniform(seed, 0, ids, 1)
+    u2 = uniform(seed, 0, ids, 2)
+    rk = math.log(kmax / kmin)
+    s = np.zeros(n, STATE_DTYPE)
+    s["charge"] = -1.0
+    s["energy"] = kmin * np.exp(rk * u0)
+    s["weight"] = s["energy"] * rk
+    ct = cmin + (cmax - cmin) * u1
+    st = np.sqrt(1.0 - ct * ct)
+    phi = 2.0 * math.pi * u2
+    s["direction"][:, 0] = st * np.cos(phi)
+    s["direction"][:, 1] = st * np.sin(phi)
+    s["direction"][:, 2] = -ct
+    s["position"][:, 2] = 1e-3
+    return s
+
+
+def slabs_backward_context(energy_loss=Mode.STRAGGLED):
+    return Context(energy_loss=energy_loss, decay=Mode.WEIGHTED,
+                   direction=Mode.BACKWARD, scattering=Mode.MIXED,
+                   event=Event.NONE)
