@@ -28,7 +28,10 @@ __all__ = [
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIBRARY_PATH = os.path.join(_HERE, "libpumas_hip.so")
+# PUMAS_HIP_LIBRARY selects another build of the same library (e.g. a
+# variant compiled with different launch parameters).
+LIBRARY_PATH = os.environ.get("PUMAS_HIP_LIBRARY",
+                              os.path.join(_HERE, "libpumas_hip.so"))
 
 
 def physics_path(name):

@@ -5,11 +5,10 @@
  * The engine is written once for the device (hipcc, gfx950). Every routine
  * that reproduces a reference routine cites it (src/pumas.c:line) and keeps
  * its floating point operation order, because per-history parity with the
- * reference is the acceptance test. Host pointers, callbacks and the
- * per-context index cache of the reference have no counterpart here:
- * table_index returns the unique bracketing row, which is what the cached
- * and uncached reference paths both return for the non decreasing tables
- * checked at export time.
+ * reference is the acceptance test. Host pointers and callbacks have no
+ * counterpart here. Table lookups return the unique bracketing row, which is
+ * what the reference's cached and uncached paths both return for the non
+ * decreasing tables checked at export time.
  */
 #ifndef PUMAS_ENGINE_COMMON_H
 #define PUMAS_ENGINE_COMMON_H
@@ -268,11 +267,51 @@ PG_FN int table_index(const double * table, int n, double value)
         return i1;
 }
 
+/*
+ * Memory of the last bracketing rows, one pair for the kinetic energy grid
+ * and one pair shared by the other grids, as the reference's per-context
+ * index cache (src/pumas.c:437-442, 4174-4210). The bracketing row of a
+ * value in a non decreasing table is unique, so the cache changes the cost
+ * of a lookup, not its result.
+ */
+struct Cache {
+        int k[2];
+        int x[2];
+
+        PG_FN void reset(int n_energies)
+        {
+                k[0] = k[1] = x[0] = x[1] = n_energies - 2;
+        }
+};
+
+PG_FN int cached_index(const double * table, int n, double value, int * last)
+{
+        if ((value >= table[last[0]]) && (value < table[last[0] + 1]))
+                return last[0];
+        if ((value >= table[last[1]]) && (value < table[last[1] + 1]))
+                return last[1];
+        const int i1 = table_index(table, n, value);
+        if ((i1 >= 0) && (i1 < n - 1)) {
+                if (i1 != last[0])
+                        last[0] = i1;
+                else if (i1 != last[1])
+                        last[1] = i1;
+        }
+        return i1;
+}
+
+/* Bracketing row in the kinetic energy grid. */
+PG_FN int index_K(const PhysicsView & p, Cache & ch, double kinetic)
+{
+        return cached_index(p.K, p.n_energies, kinetic, ch.k);
+}
+
 /* src/pumas.c:4146-4157. */
-PG_FN double interpolate(const PhysicsView & p, const double * tx,
+PG_FN double interpolate(const PhysicsView & p, Cache & ch, const double * tx,
     const double * ty, const double * tm, double x)
 {
-        const int i1 = table_index(tx, p.n_energies, x);
+        const int i1 = cached_index(
+            tx, p.n_energies, x, (tx == p.K) ? ch.k : ch.x);
         const int i2 = i1 + 1;
         const double dX = tx[i2] - tx[i1];
         const double t = (x - tx[i1]) / dX;
@@ -283,7 +322,7 @@ PG_FN double interpolate(const PhysicsView & p, const double * tx,
 
 /* cel_grammage, src/pumas.c:3644-3669. */
 PG_FN double cel_grammage(
-    const PhysicsView & p, int scheme, int material, double kinetic)
+    const PhysicsView & p, Cache & ch, int scheme, int material, double kinetic)
 {
         const int imax = p.n_energies - 1;
         if (kinetic < p.K[0]) return 0.;
@@ -295,13 +334,13 @@ PG_FN double cel_grammage(
                 return *tab2(p, p.X, scheme, material, imax) +
                     1. / b * log((kinetic + K1) / (K0 + K1));
         }
-        return interpolate(p, p.K, tab2(p, p.X, scheme, material, 0),
+        return interpolate(p, ch, p.K, tab2(p, p.X, scheme, material, 0),
             tab2(p, p.X_dK, scheme, material, 0), kinetic);
 }
 
 /* cel_grammage_as_time, src/pumas.c:3680-3709. */
 PG_FN double cel_grammage_as_time(
-    const PhysicsView & p, int scheme, int material, double time)
+    const PhysicsView & p, Cache & ch, int scheme, int material, double time)
 {
         const int imax = p.n_energies - 1;
         const double * T = tab2(p, p.T, scheme, material, 0);
@@ -318,13 +357,13 @@ PG_FN double cel_grammage_as_time(
                 return *tab2(p, p.X, scheme, material, imax) +
                     1. / b * log((kinetic + K1) / (E0 - p.mass + K1));
         }
-        return interpolate(p, T, tab2(p, p.X, scheme, material, 0),
+        return interpolate(p, ch, T, tab2(p, p.X, scheme, material, 0),
             tab2(p, p.X_dT, scheme, material, 0), time);
 }
 
 /* cel_proper_time, src/pumas.c:3720-3745. */
 PG_FN double cel_proper_time(
-    const PhysicsView & p, int scheme, int material, double kinetic)
+    const PhysicsView & p, Cache & ch, int scheme, int material, double kinetic)
 {
         const int imax = p.n_energies - 1;
         if (kinetic < p.K[0]) return 0.;
@@ -336,13 +375,13 @@ PG_FN double cel_proper_time(
                 return *tab2(p, p.T, scheme, material, imax) +
                     p.mass / a * log((E1 / E0) * (a + b * E0) / (a + b * E1));
         }
-        return interpolate(p, p.K, tab2(p, p.T, scheme, material, 0),
+        return interpolate(p, ch, p.K, tab2(p, p.T, scheme, material, 0),
             tab2(p, p.T_dK, scheme, material, 0), kinetic);
 }
 
 /* cel_kinetic_energy, src/pumas.c:3758-3784. */
 PG_FN double cel_kinetic_energy(
-    const PhysicsView & p, int scheme, int material, double grammage)
+    const PhysicsView & p, Cache & ch, int scheme, int material, double grammage)
 {
         const int imax = p.n_energies - 1;
         const double * X = tab2(p, p.X, scheme, material, 0);
@@ -355,12 +394,12 @@ PG_FN double cel_kinetic_energy(
                 return (K0 + K1) * exp(b * (grammage - X[imax])) - K1;
         }
         return interpolate(
-            p, X, p.K, tab2(p, p.K_dX, scheme, material, 0), grammage);
+            p, ch, X, p.K, tab2(p, p.K_dX, scheme, material, 0), grammage);
 }
 
 /* cel_energy_loss, src/pumas.c:3796-3814. */
 PG_FN double cel_energy_loss(
-    const PhysicsView & p, int scheme, int material, double kinetic)
+    const PhysicsView & p, Cache & ch, int scheme, int material, double kinetic)
 {
         const int imax = p.n_energies - 1;
         if (kinetic < p.K[0]) return 0.;
@@ -368,12 +407,12 @@ PG_FN double cel_energy_loss(
                 return p.a_max[material] +
                     b_max(p, scheme, material) * (kinetic + p.mass);
         }
-        return interpolate(p, p.K, tab2(p, p.dE, scheme, material, 0),
+        return interpolate(p, ch, p.K, tab2(p, p.dE, scheme, material, 0),
             tab2(p, p.dE_dK, scheme, material, 0), kinetic);
 }
 
 /* cel_straggling, src/pumas.c:3826-3846. */
-PG_FN double cel_straggling(const PhysicsView & p, int material, double kinetic)
+PG_FN double cel_straggling(const PhysicsView & p, Cache & ch, int material, double kinetic)
 {
         const int imax = p.n_energies - 1;
         if (kinetic < p.K[0]) return 0.;
@@ -382,24 +421,24 @@ PG_FN double cel_straggling(const PhysicsView & p, int material, double kinetic)
                 const double r = (kinetic + p.mass) / (Kmax + p.mass);
                 return *tab1(p, p.Omega, material, imax) * r * r;
         }
-        return interpolate(p, p.K, tab1(p, p.Omega, material, 0),
+        return interpolate(p, ch, p.K, tab1(p, p.Omega, material, 0),
             tab1(p, p.Omega_dK, material, 0), kinetic);
 }
 
 /* del_cross_section, src/pumas.c:3895-3912. */
 PG_FN double del_cross_section(
-    const PhysicsView & p, int material, double kinetic)
+    const PhysicsView & p, Cache & ch, int material, double kinetic)
 {
         const int imax = p.n_energies - 1;
         if (kinetic < p.K[0]) return 0.;
         if (kinetic >= p.K[imax]) return *tab1(p, p.CS, material, imax);
-        return interpolate(p, p.K, tab1(p, p.CS, material, 0),
+        return interpolate(p, ch, p.K, tab1(p, p.CS, material, 0),
             tab1(p, p.CS_dK, material, 0), kinetic);
 }
 
 /* del_interaction_length, src/pumas.c:3923-3947. */
 PG_FN double del_interaction_length(
-    const PhysicsView & p, int material, double kinetic)
+    const PhysicsView & p, Cache & ch, int material, double kinetic)
 {
         const int imax = p.n_energies - 1;
         if (kinetic < p.K[0]) return 0.;
@@ -413,13 +452,13 @@ PG_FN double del_interaction_length(
                         (a0 + b0 * (k0 + p.mass)));
                 return *tab1(p, p.NI_in, material, imax) + dZ;
         }
-        return interpolate(p, p.K, tab1(p, p.NI_in, material, 0),
+        return interpolate(p, ch, p.K, tab1(p, p.NI_in, material, 0),
             tab1(p, p.NI_in_dK, material, 0), kinetic);
 }
 
 /* del_kinetic_from_interaction_length, src/pumas.c:3959-3987. */
 PG_FN double del_kinetic_from_interaction_length(
-    const PhysicsView & p, int material, double nI)
+    const PhysicsView & p, Cache & ch, int material, double nI)
 {
         const int imax = p.n_energies - 1;
         const double * NI = tab1(p, p.NI_in, material, 0);
@@ -434,12 +473,12 @@ PG_FN double del_kinetic_from_interaction_length(
                            a0) / b0 - p.mass;
         }
         return interpolate(
-            p, NI, p.K, tab1(p, p.K_dNI_in, material, 0), nI);
+            p, ch, NI, p.K, tab1(p, p.K_dNI_in, material, 0), nI);
 }
 
 /* ehs_interaction_length, src/pumas.c:3998-4023. */
 PG_FN double ehs_interaction_length(
-    const PhysicsView & p, int scheme, int material, double kinetic)
+    const PhysicsView & p, Cache & ch, int scheme, int material, double kinetic)
 {
         const int imax = p.n_energies - 1;
         const double * NI = tab_NI_el(p, p.NI_el, scheme, material, 0);
@@ -451,13 +490,13 @@ PG_FN double ehs_interaction_length(
                 const double nI1 = NI[imax];
                 return nI1 + (kinetic - k1) * (nI1 - nI0) / (k1 - k0);
         }
-        return interpolate(p, p.K, NI,
+        return interpolate(p, ch, p.K, NI,
             tab_NI_el(p, p.NI_el_dK, scheme, material, 0), kinetic);
 }
 
 /* ehs_kinetic_from_interaction_length, src/pumas.c:4034-4058. */
 PG_FN double ehs_kinetic_from_interaction_length(
-    const PhysicsView & p, int scheme, int material, double nI)
+    const PhysicsView & p, Cache & ch, int scheme, int material, double nI)
 {
         const int imax = p.n_energies - 1;
         const double * NI = tab_NI_el(p, p.NI_el, scheme, material, 0);
@@ -471,11 +510,11 @@ PG_FN double ehs_kinetic_from_interaction_length(
         }
         /* K_dNI_el shares the two-scheme layout of the other CEL tables. */
         return interpolate(
-            p, NI, p.K, tab2(p, p.K_dNI_el, scheme, material, 0), nI);
+            p, ch, NI, p.K, tab2(p, p.K_dNI_el, scheme, material, 0), nI);
 }
 
 /* table_get_msc, src/pumas.c:4077-4111. */
-PG_FN void table_get_msc(const PhysicsView & p, int scheme, int material,
+PG_FN void table_get_msc(const PhysicsView & p, Cache & ch, int scheme, int material,
     double kinetic, double * mu0, double * invlb1)
 {
         const int imax = p.n_energies - 1;
@@ -489,7 +528,7 @@ PG_FN void table_get_msc(const PhysicsView & p, int scheme, int material,
                     p.K[imax] / kinetic;
         } else {
                 const double * x = p.K;
-                const int i1 = table_index(x, p.n_energies, kinetic);
+                const int i1 = index_K(p, ch, kinetic);
                 x += i1;
                 const double * y0 = tab1(p, p.Mu0, material, i1);
                 const double * m0 = tab1(p, p.Mu0_dK, material, i1);
@@ -504,7 +543,7 @@ PG_FN void table_get_msc(const PhysicsView & p, int scheme, int material,
 
 /* coulomb_ehs_length, src/pumas.c:7446-7466. */
 PG_FN double coulomb_ehs_length(
-    const PhysicsView & p, int material, double kinetic)
+    const PhysicsView & p, Cache & ch, int material, double kinetic)
 {
         const double p2 = kinetic * (kinetic + 2. * p.mass);
         const int imax = p.n_energies - 1;
@@ -514,7 +553,7 @@ PG_FN double coulomb_ehs_length(
         } else if (kinetic >= p.K[imax]) {
                 return Lb[imax] / p2;
         } else {
-                const int i1 = table_index(p.K, p.n_energies, kinetic);
+                const int i1 = index_K(p, ch, kinetic);
                 const int i2 = i1 + 1;
                 const double h = (kinetic - p.K[i1]) / (p.K[i2] - p.K[i1]);
                 return (Lb[i1] + h * (Lb[i2] - Lb[i1])) / p2;

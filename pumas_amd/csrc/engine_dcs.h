@@ -558,7 +558,7 @@ PG_FN int bracketf(const float * table, int n, float value)
  * Includes the Jacobian factor K / (K + m).
  */
 PG_FN double dcs_evaluate(const PhysicsView & p, const ModelView & mv,
-    int ip, int iel, double K, double q)
+    Cache & ch, int ip, int iel, double K, double q)
 {
         const double wj = K / (K + p.mass);
         if (ip == 3) return dcs_exact(p, mv, ip, iel, K, q) * wj;
@@ -598,7 +598,7 @@ PG_FN double dcs_evaluate(const PhysicsView & p, const ModelView & mv,
         const double Kmax = p.K[imax];
         int i0 = 0;
         if ((K >= Kmax) ||
-            ((i0 = table_index(p.K, p.n_energies, K)) >= imax)) {
+            ((i0 = index_K(p, ch, K)) >= imax)) {
                 const float * data = p.DCS +
                     (size_t)nx * 2 * ((ip * p.n_elements + eidx) * p.n_energies +
                                          imax);

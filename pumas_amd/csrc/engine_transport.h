@@ -27,10 +27,31 @@ struct Track {
         double step_X_limit, step_invlb1, lifetime;
         int randn_done;
         double randn_next;
-        /* Random stream and step counter. */
+        /* Table row memory, random stream and step counter. */
+        Cache ix;
         Rng rng;
         uint32_t steps;
         int max_steps;
+};
+
+enum {
+        PHASE_EMPTY = 0,
+        PHASE_STEP,
+        PHASE_DEL,
+        PHASE_EHS,
+        PHASE_DONE
+};
+
+/* A history in flight: its Track and the loop carried variables of the
+ * reference's stepping loop (src/pumas.c:5231-5273). */
+struct Walker {
+        Track t;
+        int phase;
+        int event, rc;
+        int media0, end_medium;
+        int medium, material, straight, step_index, step_max_type;
+        double density, step_max_medium;
+        double ti, wi, Xi, Xf, dei, grammage_max;
 };
 
 struct Engine {
@@ -146,8 +167,8 @@ struct Engine {
                 const double ki = t.energy;
                 const double di = t.distance;
                 const double ti = t.time;
-                const double xi = cel_grammage(p, MODE_CSDA, material, ki);
-                const double Ti = cel_proper_time(p, MODE_CSDA, material, ki);
+                const double xi = cel_grammage(p, t.ix, MODE_CSDA, material, ki);
+                const double Ti = cel_proper_time(p, t.ix, MODE_CSDA, material, ki);
 
                 int event = EVENT_NONE;
                 double xB;
@@ -182,8 +203,7 @@ struct Engine {
                                 const double Tf = Ti - s * dt * density;
                                 if (Tf > 0.) {
                                         const double xT = fabs(xi -
-                                            cel_grammage_as_time(
-                                                p, MODE_CSDA, material, Tf));
+                                            cel_grammage_as_time(p, t.ix, MODE_CSDA, material, Tf));
                                         if (xT < xB) {
                                                 xB = xT;
                                                 event = EVENT_LIMIT_TIME;
@@ -198,12 +218,11 @@ struct Engine {
                         if (xi > xB) {
                                 xf = xi - xB;
                                 if (xB < EPSILON_X * xi) {
-                                        kf = ki - cel_energy_loss(p, MODE_CSDA,
+                                        kf = ki - cel_energy_loss(p, t.ix, MODE_CSDA,
                                                       material, ki) * xB;
                                         if (kf < 0.) kf = 0.;
                                 } else {
-                                        kf = cel_kinetic_energy(
-                                            p, MODE_CSDA, material, xf);
+                                        kf = cel_kinetic_energy(p, t.ix, MODE_CSDA, material, xf);
                                 }
                         } else {
                                 xf = kf = 0.;
@@ -214,8 +233,7 @@ struct Engine {
                                         return EVENT_LIMIT_ENERGY;
                                 if (kf < c.limit_energy) {
                                         kf = c.limit_energy;
-                                        xf = cel_grammage(
-                                            p, MODE_CSDA, material, kf);
+                                        xf = cel_grammage(p, t.ix, MODE_CSDA, material, kf);
                                         event = EVENT_LIMIT_ENERGY;
                                 }
                         }
@@ -226,11 +244,10 @@ struct Engine {
                         } else {
                                 xf = xB + xi;
                                 if (xB < EPSILON_X * xi) {
-                                        kf = ki + cel_energy_loss(p, MODE_CSDA,
+                                        kf = ki + cel_energy_loss(p, t.ix, MODE_CSDA,
                                                       material, ki) * xB;
                                 } else {
-                                        kf = cel_kinetic_energy(
-                                            p, MODE_CSDA, material, xf);
+                                        kf = cel_kinetic_energy(p, t.ix, MODE_CSDA, material, xf);
                                 }
                         }
                         if (c.event & EVENT_LIMIT_ENERGY) {
@@ -238,8 +255,7 @@ struct Engine {
                                         return EVENT_LIMIT_ENERGY;
                                 if (kf > c.limit_energy) {
                                         kf = c.limit_energy;
-                                        xf = cel_grammage(
-                                            p, MODE_CSDA, material, kf);
+                                        xf = cel_grammage(p, t.ix, MODE_CSDA, material, kf);
                                         event = EVENT_LIMIT_ENERGY;
                                 }
                         }
@@ -260,12 +276,12 @@ struct Engine {
                         t.decayed = decayed;
                         if (decayed) event = EVENT_VERTEX_DECAY;
                 } else
-                        t.time += fabs(Ti - cel_proper_time(p, MODE_CSDA,
+                        t.time += fabs(Ti - cel_proper_time(p, t.ix, MODE_CSDA,
                                                 material, kf)) / density;
                 if (c.direction == MODE_BACKWARD)
                         t.weight *=
-                            cel_energy_loss(p, MODE_CSDA, material, kf) /
-                            cel_energy_loss(p, MODE_CSDA, material, ki);
+                            cel_energy_loss(p, t.ix, MODE_CSDA, material, kf) /
+                            cel_energy_loss(p, t.ix, MODE_CSDA, material, ki);
                 if (c.decay == MODE_WEIGHTED)
                         t.weight *= exp(-fabs(ti - t.time) / p.ctau);
 
@@ -311,7 +327,7 @@ struct Engine {
                 if (scheme == MODE_DISABLED) {
                         if (c.scattering == MODE_MIXED) {
                                 const double X = Xi -
-                                    coulomb_ehs_length(p, material, t.energy) *
+                                    coulomb_ehs_length(p, t.ix, material, t.energy) *
                                         log(t.rng.uniform());
                                 if ((*grammage_max == 0.) ||
                                     (X < *grammage_max)) {
@@ -330,12 +346,10 @@ struct Engine {
                         for (int i = 0; i < 1000; i++) {
                                 const double zeta = t.rng.uniform();
                                 if ((zeta <= 0.) || (zeta >= 1.)) continue;
-                                const double nI = del_interaction_length(
-                                    p, material, t.energy) + s * log(zeta);
+                                const double nI = del_interaction_length(p, t.ix, material, t.energy) + s * log(zeta);
                                 if (nI <= 0.) break;
                                 const double k =
-                                    del_kinetic_from_interaction_length(
-                                        p, material, nI);
+                                    del_kinetic_from_interaction_length(p, t.ix, material, nI);
                                 if (c.direction == MODE_FORWARD) {
                                         if (k >= t.energy) continue;
                                 } else {
@@ -352,13 +366,11 @@ struct Engine {
 
                 if ((scheme < MODE_STRAGGLED) &&
                     (c.scattering == MODE_MIXED)) {
-                        const double nI = ehs_interaction_length(
-                            p, scheme, material, t.energy) +
+                        const double nI = ehs_interaction_length(p, t.ix, scheme, material, t.energy) +
                             s * log(t.rng.uniform());
                         if (nI > 0.) {
                                 const double k =
-                                    ehs_kinetic_from_interaction_length(
-                                        p, scheme, material, nI);
+                                    ehs_kinetic_from_interaction_length(p, t.ix, scheme, material, nI);
                                 if ((kinetic_limit <= 0.) ||
                                     ((c.direction == MODE_FORWARD) &&
                                         (k > kinetic_limit)) ||
@@ -373,7 +385,7 @@ struct Engine {
                 if (foreseen == EVENT_NONE) return;
 
                 const double X = Xi +
-                    s * (Xf - cel_grammage(p, scheme, material, kinetic_limit));
+                    s * (Xf - cel_grammage(p, t.ix, scheme, material, kinetic_limit));
                 if ((*grammage_max <= 0) || (X < *grammage_max)) {
                         *grammage_max = X;
                         t.step_foreseen = foreseen;
@@ -387,19 +399,18 @@ struct Engine {
                 const int scheme = c.energy_loss;
                 const double s = sgn();
                 double k1, r = 0.;
-                k1 = cel_kinetic_energy(p, scheme, material, Xtot - s * dX);
+                k1 = cel_kinetic_energy(p, t.ix, scheme, material, Xtot - s * dX);
                 const double dk0 = fabs(t.energy - k1);
                 if ((k1 > 0.) && (dk0 > 0.)) {
                         const double Omega0 =
-                            cel_straggling(p, material, t.energy);
-                        const double Omega1 = cel_straggling(p, material, k1);
+                            cel_straggling(p, t.ix, material, t.energy);
+                        const double Omega1 = cel_straggling(p, t.ix, material, k1);
                         double dk12 = 0.5 * dX * (Omega0 + Omega1);
                         const double tmp = dX / Xtot;
                         if (tmp > 5E-02) {
                                 const double de1 =
-                                    cel_energy_loss(p, scheme, material, k1);
-                                const double de0 = cel_energy_loss(
-                                    p, scheme, material, t.energy);
+                                    cel_energy_loss(p, t.ix, scheme, material, k1);
+                                const double de0 = cel_energy_loss(p, t.ix, scheme, material, t.energy);
                                 dk12 *= 1. + s * dX / dk0 * (de1 - de0);
                         }
                         const double dk1 = sqrt(dk12);
@@ -453,7 +464,7 @@ struct Engine {
                 const int tmp_scheme =
                     (scheme == MODE_DISABLED) ? MODE_CSDA : scheme;
                 const double Xtot =
-                    cel_grammage(p, tmp_scheme, material, t.energy);
+                    cel_grammage(p, t.ix, tmp_scheme, material, t.energy);
 
                 double step_loc;
                 double invlb1 = 0.;
@@ -471,7 +482,7 @@ struct Engine {
                         if (c.scattering == MODE_MIXED) {
                                 if (t.step_first != 0) {
                                         double mu0;
-                                        table_get_msc(p, scheme, material,
+                                        table_get_msc(p, t.ix, scheme, material,
                                             t.energy, &mu0, &invlb1);
                                         invlb1 *= density;
                                 } else
@@ -603,7 +614,7 @@ struct Engine {
                                 }
                         } else if (dX < EPSILON_X * Xtot) {
                                 k1 -= sg *
-                                    cel_energy_loss(p, scheme, material, k1) *
+                                    cel_energy_loss(p, t.ix, scheme, material, k1) *
                                     dX;
                                 if ((c.event & EVENT_LIMIT_ENERGY) &&
                                     (c.limit_energy > 0.)) {
@@ -618,8 +629,7 @@ struct Engine {
                                         k1 = 0.;
                                 }
                         } else {
-                                k1 = cel_kinetic_energy(
-                                    p, scheme, material, X);
+                                k1 = cel_kinetic_energy(p, t.ix, scheme, material, X);
                         }
                 } else if (scheme == MODE_STRAGGLED) {
                         double ratio;
@@ -660,21 +670,20 @@ struct Engine {
                         /* In-step inelastic DEL. */
                         do {
                                 double xs_del =
-                                    del_cross_section(p, material, t.energy);
+                                    del_cross_section(p, t.ix, material, t.energy);
                                 const double tmp_del =
-                                    del_cross_section(p, material, k1);
+                                    del_cross_section(p, t.ix, material, k1);
                                 if (tmp_del > xs_del) xs_del = tmp_del;
                                 if (xs_del <= 0.) break;
                                 const double X_del =
                                     -log(t.rng.uniform()) / xs_del;
                                 if (X_del >= Xmax) break;
-                                const double k_h = cel_kinetic_energy(
-                                    p, scheme, material, Xtot - sg * X_del);
+                                const double k_h = cel_kinetic_energy(p, t.ix, scheme, material, Xtot - sg * X_del);
                                 const double k_del = t.energy -
                                     sg * ratio * fabs(t.energy - k_h);
                                 if (k_del <= 0.) break;
                                 const double r =
-                                    del_cross_section(p, material, k_del) /
+                                    del_cross_section(p, t.ix, material, k_del) /
                                     xs_del;
                                 if (t.rng.uniform() > r) break;
                                 k_x = k_del;
@@ -694,18 +703,17 @@ struct Engine {
                                         kmin = kmax, kmax = tmp;
                                 }
                                 const double lb_ehs =
-                                    coulomb_ehs_length(p, material, kmin);
+                                    coulomb_ehs_length(p, t.ix, material, kmin);
                                 if (lb_ehs <= 0.) break;
                                 const double X_ehs =
                                     -log(t.rng.uniform()) * lb_ehs;
                                 if (X_ehs >= Xmax) break;
-                                const double k_h = cel_kinetic_energy(
-                                    p, scheme, material, Xtot - sg * X_ehs);
+                                const double k_h = cel_kinetic_energy(p, t.ix, scheme, material, Xtot - sg * X_ehs);
                                 const double k_ehs = t.energy -
                                     sg * ratio * fabs(t.energy - k_h);
                                 if (k_ehs <= 0.) break;
                                 const double r =
-                                    coulomb_ehs_length(p, material, k_ehs) /
+                                    coulomb_ehs_length(p, t.ix, material, k_ehs) /
                                     lb_ehs;
                                 if ((r <= 0.) || (t.rng.uniform() > 1. / r))
                                         break;
@@ -740,7 +748,7 @@ struct Engine {
                         if (!(event & EVENT_LIMIT_ENERGY)) {
                                 if (scheme <= MODE_MIXED) {
                                         if (scheme != MODE_DISABLED)
-                                                k1 = cel_kinetic_energy(p,
+                                                k1 = cel_kinetic_energy(p, t.ix,
                                                     scheme, material,
                                                     Xtot - sg * (t.grammage -
                                                                     Xi));
@@ -773,14 +781,13 @@ struct Engine {
                 const double sf1 = step;
                 if (straight && (scheme != MODE_DISABLED)) {
                         const double Ti =
-                            cel_proper_time(p, scheme, material, t.energy);
+                            cel_proper_time(p, t.ix, scheme, material, t.energy);
                         if (time_max > 0.) {
                                 const double Tf = Ti -
                                     sg * (time_max - t.time) * density;
                                 if (Tf > 0.) {
                                         const double dxT = fabs(Xtot -
-                                            cel_grammage_as_time(
-                                                p, scheme, material, Tf));
+                                            cel_grammage_as_time(p, t.ix, scheme, material, Tf));
                                         if (Xi + dxT < t.grammage) {
                                                 event = EVENT_LIMIT_TIME;
                                                 t.time = time_max;
@@ -790,14 +797,13 @@ struct Engine {
                                                 t.grammage = Xi + dxT;
                                                 const double xf =
                                                     Xtot - sg * dxT;
-                                                k1 = cel_kinetic_energy(
-                                                    p, scheme, material, xf);
+                                                k1 = cel_kinetic_energy(p, t.ix, scheme, material, xf);
                                         }
                                 }
                         }
                         if (event != EVENT_LIMIT_TIME) {
                                 const double Tf =
-                                    cel_proper_time(p, scheme, material, k1);
+                                    cel_proper_time(p, t.ix, scheme, material, k1);
                                 t.time += fabs(Tf - Ti) * density_i;
                         }
                 } else {
@@ -860,7 +866,7 @@ struct Engine {
                         if (t.energy <= 0.) {
                                 t.step_invlb1 = invlb1;
                         } else {
-                                table_get_msc(p, scheme, material, t.energy,
+                                table_get_msc(p, t.ix, scheme, material, t.energy,
                                     &mu0, &invlb1_);
                                 t.step_invlb1 = locals_density * invlb1_;
                         }
@@ -893,7 +899,7 @@ struct Engine {
         {
                 int i1, i2;
                 double h, dk;
-                i1 = table_index(p.K, p.n_energies, t.energy);
+                i1 = index_K(p, t.ix, t.energy);
                 if (i1 < 0) {
                         i1 = i2 = 0;
                         h = dk = 0.;
@@ -950,8 +956,7 @@ struct Engine {
                         for (int ip = 0; ip < N_DEL_PROCESSES; ip++) {
                                 for (int ic = ic0; ic < ic0 + nc; ic++) {
                                         const int iel = p.component_element[ic];
-                                        const double d = dcs_evaluate(
-                                            p, mv, ip, iel, t.energy, Q);
+                                        const double d = dcs_evaluate(p, mv, t.ix, ip, iel, t.energy, Q);
                                         stot += d * p.component_fraction[ic];
                                 }
                         }
@@ -984,7 +989,7 @@ struct Engine {
                         for (int ic = ic0; ic < ic0 + nc; ic++)
                                 for (int ip = 0; ip < N_DEL_PROCESSES; ip++) {
                                         const int iel = p.component_element[ic];
-                                        const double si = dcs_evaluate(p, mv,
+                                        const double si = dcs_evaluate(p, mv, t.ix,
                                             ip, iel, t.energy, Q2) *
                                             p.component_fraction[ic];
                                         s += si;
@@ -1082,7 +1087,7 @@ struct Engine {
                         p0 = 1.;
                         a1 = b1 = 0.;
                 } else {
-                        const int row = table_index(p.K, p.n_energies, t.energy);
+                        const int row = index_K(p, t.ix, t.energy);
                         const float * envelope0 = p.DCS_envelope +
                             2 * ((process * p.n_elements + eidx) *
                                         p.n_energies + row);
@@ -1127,8 +1132,7 @@ struct Engine {
 
                         const double penv =
                             b0 * pow(x, -a0) + b1 * pow(x, -a1);
-                        const double d = dcs_evaluate(
-                            p, mv, process, iel, t.energy, t.energy * x);
+                        const double d = dcs_evaluate(p, mv, t.ix, process, iel, t.energy, t.energy * x);
                         if (t.rng.uniform() * penv <= d) break;
                 }
                 if (i == 100) return 0;
@@ -1202,7 +1206,7 @@ struct Engine {
                 }
 
                 t.weight *= w_bias * weight /
-                    (del_cross_section(p, material, kf) * (1. - pCEL));
+                    (del_cross_section(p, t.ix, material, kf) * (1. - pCEL));
                 *process_out = process;
 
                 if (c.scattering == MODE_DISABLED) return -1;
@@ -1235,13 +1239,12 @@ struct Engine {
                         else
                                 t.step_event = EVENT_WEIGHT;
                 } else {
-                        const int event_for[N_DEL_PROCESSES + 1] = {
-                                EVENT_VERTEX_BREMSSTRAHLUNG,
-                                EVENT_VERTEX_PAIR_CREATION,
-                                EVENT_VERTEX_PHOTONUCLEAR,
-                                EVENT_VERTEX_DELTA_RAY, EVENT_VERTEX
-                        };
-                        t.step_event = event_for[process];
+                        t.step_event = (process == 0) ?
+                            EVENT_VERTEX_BREMSSTRAHLUNG :
+                            (process == 1) ? EVENT_VERTEX_PAIR_CREATION :
+                            (process == 2) ? EVENT_VERTEX_PHOTONUCLEAR :
+                            (process == 3) ? EVENT_VERTEX_DELTA_RAY :
+                                             EVENT_VERTEX;
                 }
 
                 if ((c.scattering == MODE_MIXED) && (polar >= 0)) {
@@ -1256,35 +1259,47 @@ struct Engine {
                 }
         }
 
-        /* transport_do_ehs, src/pumas.c:5768-5885. */
+        /* Coulomb scattering data of component i of a material, as set up
+         * per element by transport_do_ehs (src/pumas.c:5786-5815). */
+        PG_FN void coulomb_setup(CoulombData & d, int material, int i,
+            double kinetic, double mu0) const
+        {
+                const int ic = component0(material) + i;
+                const int iel = p.component_element[ic];
+                const double Z = p.element_Z[iel];
+                const double A = p.element_A[iel];
+                double kinetic0;
+                coulomb_frame_parameters(
+                    Z, A, p.mass, kinetic, &kinetic0, d.fCM);
+                d.fspin = coulomb_spin_factor(p.mass, kinetic);
+                coulomb_screening_parameters(p, iel, kinetic, kinetic0,
+                    &d.n_parameters, d.amplitude, d.screening);
+                coulomb_pole_reduction(d);
+                d.normalisation = p.component_fraction[ic] /
+                    coulomb_normalisation(Z, A, p.mass, kinetic, kinetic0);
+                d.cs_hard = d.normalisation * coulomb_restricted_cs(mu0, d);
+        }
+
+        /*
+         * transport_do_ehs, src/pumas.c:5768-5885. The per element data of
+         * the reference's workspace are recomputed for the selected element
+         * instead of being kept for all elements; the recomputation repeats
+         * the same operations on the same inputs.
+         */
         PG_NOINLINE void transport_do_ehs(Track & t, int material) const
         {
                 const double kinetic = t.energy;
                 double mu0 = 0., invlb1 = 0.;
-                table_get_msc(p, c.energy_loss, material, kinetic, &mu0,
+                table_get_msc(p, t.ix, c.energy_loss, material, kinetic, &mu0,
                     &invlb1);
 
                 const int nc = p.elements_in[material];
-                const int ic0 = component0(material);
-                CoulombData data[MAX_COMPONENTS];
+                double cs_hard[MAX_COMPONENTS];
                 double cs_tot = 0.;
                 for (int i = 0; i < nc; i++) {
-                        CoulombData & d = data[i];
-                        const int iel = p.component_element[ic0 + i];
-                        const double Z = p.element_Z[iel];
-                        const double A = p.element_A[iel];
-                        double kinetic0;
-                        coulomb_frame_parameters(
-                            Z, A, p.mass, kinetic, &kinetic0, d.fCM);
-                        d.fspin = coulomb_spin_factor(p.mass, kinetic);
-                        coulomb_screening_parameters(p, iel, kinetic, kinetic0,
-                            &d.n_parameters, d.amplitude, d.screening);
-                        coulomb_pole_reduction(d);
-                        d.normalisation = p.component_fraction[ic0 + i] /
-                            coulomb_normalisation(
-                                Z, A, p.mass, kinetic, kinetic0);
-                        d.cs_hard = d.normalisation *
-                            coulomb_restricted_cs(mu0, d);
+                        CoulombData d;
+                        coulomb_setup(d, material, i, kinetic, mu0);
+                        cs_hard[i] = d.cs_hard;
                         cs_tot += d.cs_hard;
                 }
 
@@ -1292,14 +1307,15 @@ struct Engine {
                 int ihard = nc - 1;
                 double cs = 0.;
                 for (int i = 0; i < nc; i++) {
-                        cs += data[i].cs_hard;
+                        cs += cs_hard[i];
                         if (cs >= cc0) {
                                 ihard = i;
                                 break;
                         }
                 }
 
-                const CoulombData & d = data[ihard];
+                CoulombData d;
+                coulomb_setup(d, material, ihard, kinetic, mu0);
                 const int n = d.n_parameters - 1;
                 double A = d.screening[0];
                 for (int i = 1; i < n; i++) {
@@ -1343,195 +1359,58 @@ struct Engine {
                 rotate(t, mu);
         }
 
-        /* transport_with_stepping, src/pumas.c:5194-5514. */
-        PG_NOINLINE int transport_with_stepping(Track & t, int * medium_ptr,
-            double density, double step_max_medium, int step_max_type,
-            double step_max_locals) const
-        {
-                int medium = *medium_ptr;
-                int material;
-                density = locals_density(medium, &material);
-
-                const int scheme = c.energy_loss;
-                int straight = ((c.scattering == MODE_DISABLED) &&
-                                   (scheme <= MODE_MIXED) &&
-                                   (step_max_locals <= 0.)) ?
-                    1 :
-                    0;
-
-                t.step_event = EVENT_NONE;
-
-                double ti = t.time;
-                double wi = t.weight;
-                double Xi = t.grammage;
-                double dei, Xf;
-                if (scheme > MODE_DISABLED) {
-                        const double ki = t.energy;
-                        Xf = cel_grammage(p, scheme, material, ki);
-                        dei = 1. / cel_energy_loss(p, scheme, material, ki);
-                } else {
-                        Xf = dei = 0.;
-                }
-
-                if ((c.event & EVENT_LIMIT_DISTANCE) &&
-                    (t.distance >= c.limit_distance))
-                        return EVENT_LIMIT_DISTANCE;
-                if ((c.event & EVENT_LIMIT_GRAMMAGE) &&
-                    (t.grammage >= c.limit_grammage))
-                        return EVENT_LIMIT_GRAMMAGE;
-                if ((c.event & EVENT_LIMIT_TIME) && (t.time >= c.limit_time))
-                        return EVENT_LIMIT_TIME;
-                if (t.weight <= 0.) return EVENT_WEIGHT;
-
-                double grammage_max = 0.;
-                t.step_event = EVENT_NONE;
-                t.step_first = 1;
-                t.step_X_limit = (c.event & EVENT_LIMIT_ENERGY) ?
-                    cel_grammage(p,
-                        (scheme > MODE_DISABLED) ? scheme : MODE_CSDA,
-                        material, c.limit_energy) :
-                    0.;
-                t.step_invlb1 = 0;
-                transport_limit(t, material, Xi, Xf, &grammage_max);
-                if (t.step_event) return t.step_event;
-
-                int step_index = 1;
-                for (;;) {
-                        if (step_index > 1) {
-                                int unused;
-                                step_max_type =
-                                    medium_step(t, &unused, &step_max_medium);
-                                if ((step_max_medium > 0.) &&
-                                    (step_max_type == STEP_CHECK))
-                                        step_max_medium += 0.5 * STEP_MIN;
-                        }
-                        if ((int)t.steps >= t.max_steps) {
-                                /* Safety cap: end the history as a limit. */
-                                t.step_event = EVENT_LIMIT;
-                                break;
-                        }
-
-                        int new_medium = -1;
-                        step_transport(t, straight, medium, material, density,
-                            grammage_max, step_max_medium, step_max_type,
-                            &new_medium);
-                        step_index++;
-
-                        if (!t.step_event) continue;
-
-                        if ((c.direction == MODE_BACKWARD) &&
-                            (c.energy_loss >= MODE_CSDA)) {
-                                const double wt = (c.decay == MODE_WEIGHTED) ?
-                                    exp(-fabs(t.time - ti) / p.ctau) :
-                                    1.;
-                                t.weight = wi * wt * dei *
-                                    cel_energy_loss(
-                                        p, scheme, material, t.energy);
-                        } else if (c.decay == MODE_WEIGHTED) {
-                                t.weight = wi * exp(-fabs(t.time - ti) / p.ctau);
-                        }
-
-                        if (t.step_event & (EVENT_LIMIT | EVENT_WEIGHT |
-                                               EVENT_VERTEX_DECAY)) {
-                                break;
-                        } else if (t.step_event &
-                            (EVENT_VERTEX_DEL | EVENT_VERTEX_COULOMB)) {
-                                if (t.step_event & EVENT_VERTEX_DEL) {
-                                        transport_do_del(t, material);
-                                        if ((c.event & t.step_event) ||
-                                            t.step_event == EVENT_WEIGHT)
-                                                break;
-                                        t.step_first = 1;
-                                        t.step_invlb1 = 0;
-                                } else {
-                                        transport_do_ehs(t, material);
-                                        if (c.event & t.step_event) break;
-                                }
-                        } else if (t.step_event & EVENT_MEDIUM) {
-                                medium = new_medium;
-                                if ((medium < 0) || (c.event & EVENT_MEDIUM))
-                                        break;
-                                density = locals_density(medium, &material);
-                                straight = ((c.scattering == MODE_DISABLED) &&
-                                               (scheme <= MODE_MIXED)) ?
-                                    1 :
-                                    0;
-                                const int tmp_scheme =
-                                    scheme > MODE_DISABLED ? scheme : MODE_CSDA;
-                                t.step_X_limit =
-                                    (c.event & EVENT_LIMIT_ENERGY) ?
-                                    cel_grammage(p, tmp_scheme, material,
-                                        c.limit_energy) :
-                                    0.;
-                                t.step_first = 1;
-                                t.step_invlb1 = 0;
-                        }
-
-                        ti = t.time;
-                        wi = t.weight;
-                        Xi = t.grammage;
-                        if (c.energy_loss >= MODE_CSDA) {
-                                const double ki = t.energy;
-                                Xf = cel_grammage(p, scheme, material, ki);
-                                dei = 1. /
-                                    cel_energy_loss(p, scheme, material, ki);
-                        }
-                        transport_limit(t, material, Xi, Xf, &grammage_max);
-                        if (t.step_event) break;
-                }
-
-                if (c.direction == MODE_FORWARD) {
-                        const double kinetic_min =
-                            (c.event & EVENT_LIMIT_ENERGY) ? c.limit_energy :
-                                                             0.;
-                        if (fabs(t.energy - kinetic_min) < FLT_EPS)
-                                t.energy = kinetic_min;
-                } else {
-                        if ((c.event & EVENT_LIMIT_ENERGY) &&
-                            (fabs(t.energy - c.limit_energy) < FLT_EPS))
-                                t.energy = c.limit_energy;
-                }
-
-                *medium_ptr = medium;
-                return t.step_event;
-        }
-
         /*
-         * pumas_context_transport, src/pumas.c:3114-3251, for one history.
-         * Batch level checks (accuracy, cutoff, decay mode) are done by the
-         * host before the launch. Returns the enum pumas_return code.
+         * The stepping transport of transport_with_stepping
+         * (src/pumas.c:5194-5514) and the entry logic of
+         * pumas_context_transport (src/pumas.c:3114-3251), cut at the
+         * points where the reference's step loop branches on an event, so
+         * that a history advances one phase at a time: INIT (entry checks,
+         * closed form CSDA, stepping set up), STEP (one Monte Carlo step),
+         * DEL and EHS (a pending discrete event), DONE. The operations and
+         * random draws of a history are the reference's, in its order; a
+         * scheduler may interleave the phases of many histories freely.
          */
-        PG_FN int transport(Track & t, int * event, int * media) const
+        PG_NOINLINE void walker_init(Walker & w) const
         {
-                *event = EVENT_NONE;
-                media[0] = media[1] = -1;
+                Track & t = w.t;
+                w.event = EVENT_NONE;
+                w.rc = RETURN_SUCCESS;
+                w.media0 = w.medium = -1;
+                w.phase = PHASE_DONE;
                 if (t.decayed) {
-                        *event = EVENT_VERTEX_DECAY;
-                        return RETURN_SUCCESS;
+                        w.event = EVENT_VERTEX_DECAY;
+                        w.end_medium = -1;
+                        return;
                 }
                 {
                         const double * const u = t.dir;
                         const double norm2 =
                             u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-                        if (fabs(norm2 - 1) > FLT_EPS)
-                                return RETURN_DIRECTION_ERROR;
+                        if (fabs(norm2 - 1) > FLT_EPS) {
+                                w.rc = RETURN_DIRECTION_ERROR;
+                                w.end_medium = -1;
+                                return;
+                        }
                 }
 
                 int medium;
                 double step_max_medium;
                 int step_max_type = medium_step(t, &medium, &step_max_medium);
-                media[0] = medium;
-                media[1] = -1;
+                w.media0 = medium;
+                w.end_medium = -1;
                 if (medium < 0) {
-                        *event = EVENT_MEDIUM;
-                        return RETURN_SUCCESS;
+                        w.event = EVENT_MEDIUM;
+                        return;
                 } else if ((step_max_medium > 0.) &&
                     (step_max_type == STEP_CHECK))
                         step_max_medium += 0.5 * STEP_MIN;
                 int material;
                 const double density = locals_density(medium, &material);
                 const double step_max_locals = 0.;
-                if (density <= 0.) return RETURN_DENSITY_ERROR;
+                if (density <= 0.) {
+                        w.rc = RETURN_DENSITY_ERROR;
+                        return;
+                }
 
                 if (c.decay == MODE_RANDOMISED) {
                         const double u = t.rng.uniform();
@@ -1543,23 +1422,235 @@ struct Engine {
                     (c.energy_loss <= MODE_CSDA)) {
                         if ((c.energy_loss == MODE_DISABLED) &&
                             ((c.event & EVENT_LIMIT) == 0)) {
-                                return RETURN_MISSING_LIMIT;
+                                w.rc = RETURN_MISSING_LIMIT;
+                                return;
                         } else if ((c.scattering == MODE_DISABLED) &&
                             (c.energy_loss == MODE_CSDA)) {
                                 do_stepping = 0;
                         }
                 }
-
-                int e;
-                if (do_stepping) {
-                        e = transport_with_stepping(t, &medium, density,
-                            step_max_medium, step_max_type, step_max_locals);
-                } else {
-                        e = transport_with_csda(t, material, density);
+                /* The end medium of a history that stops before stepping
+                 * is its start medium. */
+                w.end_medium = medium;
+                if (!do_stepping) {
+                        w.event = transport_with_csda(t, material, density);
+                        return;
                 }
-                *event = e;
-                media[1] = medium;
-                return RETURN_SUCCESS;
+
+                /* transport_with_stepping set up, src/pumas.c:5210-5273. */
+                w.medium = medium;
+                w.material = material;
+                w.density = density;
+                w.step_max_medium = step_max_medium;
+                w.step_max_type = step_max_type;
+                const int scheme = c.energy_loss;
+                w.straight = ((c.scattering == MODE_DISABLED) &&
+                                 (scheme <= MODE_MIXED) &&
+                                 (step_max_locals <= 0.)) ?
+                    1 :
+                    0;
+                t.step_event = EVENT_NONE;
+                w.ti = t.time;
+                w.wi = t.weight;
+                w.Xi = t.grammage;
+                if (scheme > MODE_DISABLED) {
+                        const double ki = t.energy;
+                        w.Xf = cel_grammage(p, t.ix, scheme, material, ki);
+                        w.dei = 1. /
+                            cel_energy_loss(p, t.ix, scheme, material, ki);
+                } else {
+                        w.Xf = w.dei = 0.;
+                }
+
+                if ((c.event & EVENT_LIMIT_DISTANCE) &&
+                    (t.distance >= c.limit_distance)) {
+                        w.event = EVENT_LIMIT_DISTANCE;
+                        return;
+                }
+                if ((c.event & EVENT_LIMIT_GRAMMAGE) &&
+                    (t.grammage >= c.limit_grammage)) {
+                        w.event = EVENT_LIMIT_GRAMMAGE;
+                        return;
+                }
+                if ((c.event & EVENT_LIMIT_TIME) &&
+                    (t.time >= c.limit_time)) {
+                        w.event = EVENT_LIMIT_TIME;
+                        return;
+                }
+                if (t.weight <= 0.) {
+                        w.event = EVENT_WEIGHT;
+                        return;
+                }
+
+                w.grammage_max = 0.;
+                t.step_event = EVENT_NONE;
+                t.step_first = 1;
+                t.step_X_limit = (c.event & EVENT_LIMIT_ENERGY) ?
+                    cel_grammage(p, t.ix,
+                        (scheme > MODE_DISABLED) ? scheme : MODE_CSDA,
+                        material, c.limit_energy) :
+                    0.;
+                t.step_invlb1 = 0;
+                transport_limit(t, material, w.Xi, w.Xf, &w.grammage_max);
+                if (t.step_event) {
+                        w.event = t.step_event;
+                        return;
+                }
+                w.step_index = 1;
+                w.phase = PHASE_STEP;
+        }
+
+        /* End of the step loop, src/pumas.c:5492-5513. */
+        PG_FN void walker_finish(Walker & w) const
+        {
+                Track & t = w.t;
+                if (c.direction == MODE_FORWARD) {
+                        const double kinetic_min =
+                            (c.event & EVENT_LIMIT_ENERGY) ? c.limit_energy :
+                                                             0.;
+                        if (fabs(t.energy - kinetic_min) < FLT_EPS)
+                                t.energy = kinetic_min;
+                } else {
+                        if ((c.event & EVENT_LIMIT_ENERGY) &&
+                            (fabs(t.energy - c.limit_energy) < FLT_EPS))
+                                t.energy = c.limit_energy;
+                }
+                w.end_medium = w.medium;
+                w.event = t.step_event;
+                w.phase = PHASE_DONE;
+        }
+
+        /* Common tail of an event: new initial conditions and limits,
+         * src/pumas.c:5473-5488. */
+        PG_FN void walker_post(Walker & w) const
+        {
+                Track & t = w.t;
+                const int scheme = c.energy_loss;
+                w.ti = t.time;
+                w.wi = t.weight;
+                w.Xi = t.grammage;
+                if (c.energy_loss >= MODE_CSDA) {
+                        const double ki = t.energy;
+                        w.Xf = cel_grammage(p, t.ix, scheme, w.material, ki);
+                        w.dei = 1. /
+                            cel_energy_loss(p, t.ix, scheme, w.material, ki);
+                }
+                transport_limit(t, w.material, w.Xi, w.Xf, &w.grammage_max);
+                if (t.step_event)
+                        walker_finish(w);
+                else
+                        w.phase = PHASE_STEP;
+        }
+
+        /* One iteration of the step loop up to the event dispatch,
+         * src/pumas.c:5277-5467. */
+        PG_NOINLINE void walker_step(Walker & w) const
+        {
+                Track & t = w.t;
+                const int scheme = c.energy_loss;
+                if (w.step_index > 1) {
+                        int unused;
+                        w.step_max_type =
+                            medium_step(t, &unused, &w.step_max_medium);
+                        if ((w.step_max_medium > 0.) &&
+                            (w.step_max_type == STEP_CHECK))
+                                w.step_max_medium += 0.5 * STEP_MIN;
+                }
+                if ((int)t.steps >= t.max_steps) {
+                        /* Safety cap: end the history as a limit. */
+                        t.step_event = EVENT_LIMIT;
+                        walker_finish(w);
+                        return;
+                }
+
+                int new_medium = -1;
+                step_transport(t, w.straight, w.medium, w.material, w.density,
+                    w.grammage_max, w.step_max_medium, w.step_max_type,
+                    &new_medium);
+                w.step_index++;
+                if (!t.step_event) return;
+
+                if ((c.direction == MODE_BACKWARD) &&
+                    (c.energy_loss >= MODE_CSDA)) {
+                        const double wt = (c.decay == MODE_WEIGHTED) ?
+                            exp(-fabs(t.time - w.ti) / p.ctau) :
+                            1.;
+                        t.weight = w.wi * wt * w.dei *
+                            cel_energy_loss(p, t.ix, scheme, w.material,
+                                t.energy);
+                } else if (c.decay == MODE_WEIGHTED) {
+                        t.weight = w.wi * exp(-fabs(t.time - w.ti) / p.ctau);
+                }
+
+                if (t.step_event & (EVENT_LIMIT | EVENT_WEIGHT |
+                                       EVENT_VERTEX_DECAY)) {
+                        walker_finish(w);
+                } else if (t.step_event & EVENT_VERTEX_DEL) {
+                        w.phase = PHASE_DEL;
+                } else if (t.step_event & EVENT_VERTEX_COULOMB) {
+                        w.phase = PHASE_EHS;
+                } else if (t.step_event & EVENT_MEDIUM) {
+                        w.medium = new_medium;
+                        if ((w.medium < 0) || (c.event & EVENT_MEDIUM)) {
+                                walker_finish(w);
+                                return;
+                        }
+                        w.density = locals_density(w.medium, &w.material);
+                        w.straight = ((c.scattering == MODE_DISABLED) &&
+                                         (scheme <= MODE_MIXED)) ?
+                            1 :
+                            0;
+                        const int tmp_scheme =
+                            scheme > MODE_DISABLED ? scheme : MODE_CSDA;
+                        t.step_X_limit = (c.event & EVENT_LIMIT_ENERGY) ?
+                            cel_grammage(p, t.ix, tmp_scheme, w.material,
+                                c.limit_energy) :
+                            0.;
+                        t.step_first = 1;
+                        t.step_invlb1 = 0;
+                        walker_post(w);
+                }
+        }
+
+        /* A pending inelastic DEL, src/pumas.c:5339-5386. */
+        PG_NOINLINE void walker_del(Walker & w) const
+        {
+                Track & t = w.t;
+                transport_do_del(t, w.material);
+                if ((c.event & t.step_event) ||
+                    t.step_event == EVENT_WEIGHT) {
+                        walker_finish(w);
+                        return;
+                }
+                t.step_first = 1;
+                t.step_invlb1 = 0;
+                walker_post(w);
+        }
+
+        /* A pending elastic hard scattering, src/pumas.c:5393-5398. */
+        PG_NOINLINE void walker_ehs(Walker & w) const
+        {
+                Track & t = w.t;
+                transport_do_ehs(t, w.material);
+                if (c.event & t.step_event) {
+                        walker_finish(w);
+                        return;
+                }
+                walker_post(w);
+        }
+
+        /* Run the phases of one history to the end. */
+        PG_FN void walker_run(Walker & w) const
+        {
+                walker_init(w);
+                while (w.phase != PHASE_DONE) {
+                        if (w.phase == PHASE_STEP)
+                                walker_step(w);
+                        else if (w.phase == PHASE_DEL)
+                                walker_del(w);
+                        else
+                                walker_ehs(w);
+                }
         }
 };
 

@@ -490,12 +490,11 @@ struct KernelArgs {
         unsigned long long * counters; /* [0] queue, [1] steps, [2] error */
 };
 
-__device__ static void run_history(const KernelArgs & a, uint64_t i,
-    unsigned long long & steps_acc)
+/* Load history i into a walker. */
+__device__ static void walker_load(const KernelArgs & a, uint64_t i, Walker & w)
 {
-        const Engine eng = { a.p, a.mv, a.c, a.g };
-        struct pumas_state * s = a.states + i;
-        Track t;
+        const struct pumas_state * s = a.states + i;
+        Track & t = w.t;
         t.charge = s->charge;
         t.energy = s->energy;
         t.distance = s->distance;
@@ -514,13 +513,18 @@ __device__ static void run_history(const KernelArgs & a, uint64_t i,
         t.step_X_limit = t.step_invlb1 = t.lifetime = 0.;
         t.randn_done = 0;
         t.randn_next = 0.;
+        t.ix.reset(a.p.n_energies);
         t.rng.init(a.seed, a.first + i, STREAM_TRANSPORT, 0);
         t.steps = 0;
         t.max_steps = a.max_steps;
+}
 
-        int event, media[2];
-        const int rc = eng.transport(t, &event, media);
-
+/* Store the end state and outputs of history i. */
+__device__ static void walker_store(const KernelArgs & a, uint64_t i,
+    const Walker & w)
+{
+        struct pumas_state * s = a.states + i;
+        const Track & t = w.t;
         s->charge = t.charge;
         s->energy = t.energy;
         s->distance = t.distance;
@@ -534,37 +538,87 @@ __device__ static void run_history(const KernelArgs & a, uint64_t i,
         s->direction[1] = t.dir[1];
         s->direction[2] = t.dir[2];
         s->decayed = t.decayed;
-        if (a.events) a.events[i] = (enum pumas_event)event;
+        if (a.events) a.events[i] = (enum pumas_event)w.event;
         if (a.media) {
-                a.media[2 * i] = media[0];
-                a.media[2 * i + 1] = media[1];
+                a.media[2 * i] = w.media0;
+                a.media[2 * i + 1] = w.end_medium;
         }
         if (a.steps) a.steps[i] = t.steps;
         if (a.draws) a.draws[i] = t.rng.count;
-        if (a.rc) a.rc[i] = rc;
-        if (rc != RETURN_SUCCESS)
-                atomicCAS(a.counters + 2, 0ull, (unsigned long long)rc);
-        steps_acc += t.steps;
+        if (a.rc) a.rc[i] = w.rc;
+        if (w.rc != RETURN_SUCCESS)
+                atomicCAS(a.counters + 2, 0ull, (unsigned long long)w.rc);
 }
 
+/* Minimum number of lanes of a wavefront with a pending elastic hard
+ * scattering before the wavefront processes them together. */
+#ifndef EHS_BATCH
+#define EHS_BATCH 16
+#endif
+
 /*
- * Persistent transport kernel. Each wavefront loops: lanes without a
- * history take the next ids from the global queue with a single atomic per
- * wavefront, then every lane transports its history to completion.
+ * Persistent transport kernel. A wavefront keeps 64 histories in flight.
+ * Every iteration, lanes whose history ended take the next history ids from
+ * the global queue (one atomic per wavefront: ballot of the free lanes,
+ * prefix count for the ids), then the wavefront advances every history by
+ * one phase. Elastic hard scatterings, on ~16% of the steps of a muon in
+ * rock, are deferred until EHS_BATCH lanes are waiting for one (or no lane
+ * can step), so that the wavefront runs that code path with most of its
+ * lanes active. Each history still sees its own operations in the
+ * reference's order; only the interleaving across lanes changes.
  */
-__global__ void __launch_bounds__(256) transport_persistent(KernelArgs a)
+__global__ void __launch_bounds__(256, 2) transport_persistent(KernelArgs a)
 {
+        const Engine eng = { a.p, a.mv, a.c, a.g };
         const int lane = threadIdx.x & 63;
+        const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        Walker w;
+        w.phase = PHASE_EMPTY;
+        uint64_t index = 0;
+        bool exhausted = false;
         unsigned long long steps_acc = 0;
         for (;;) {
-                unsigned long long base = 0;
-                if (lane == 0) base = atomicAdd(a.counters, 64ull);
-                base = __shfl(base, 0, 64);
-                if (base >= a.n) break;
-                const uint64_t i = base + lane;
-                if (i < a.n) run_history(a, i, steps_acc);
+                if (w.phase == PHASE_DONE) {
+                        walker_store(a, index, w);
+                        steps_acc += w.t.steps;
+                        w.phase = PHASE_EMPTY;
+                }
+                if (!exhausted) {
+                        const uint64_t free_lanes =
+                            __ballot(w.phase == PHASE_EMPTY);
+                        if (free_lanes) {
+                                const int count = __popcll(free_lanes);
+                                const int leader = __ffsll(
+                                    (unsigned long long)free_lanes) - 1;
+                                unsigned long long base = 0;
+                                if (lane == leader)
+                                        base = atomicAdd(a.counters, count);
+                                base = __shfl(base, leader, 64);
+                                if (base + count >= a.n) exhausted = true;
+                                if (w.phase == PHASE_EMPTY) {
+                                        index = base +
+                                            __popcll(free_lanes & below);
+                                        if (index < a.n) {
+                                                walker_load(a, index, w);
+                                                eng.walker_init(w);
+                                        }
+                                }
+                        }
+                }
+                if (!__ballot(w.phase != PHASE_EMPTY)) {
+                        if (exhausted) break;
+                        continue;
+                }
+
+                if (w.phase == PHASE_STEP) eng.walker_step(w);
+                if (w.phase == PHASE_DEL) eng.walker_del(w);
+
+                const uint64_t ehs = __ballot(w.phase == PHASE_EHS);
+                const uint64_t stepping = __ballot(w.phase == PHASE_STEP);
+                if (ehs && ((__popcll(ehs) >= EHS_BATCH) || !stepping)) {
+                        if (w.phase == PHASE_EHS) eng.walker_ehs(w);
+                }
         }
-        /* Wavefront reduction of the step count, one atomic per wave. */
         for (int off = 32; off > 0; off >>= 1)
                 steps_acc += __shfl_down(steps_acc, off, 64);
         if (lane == 0) atomicAdd(a.counters + 1, steps_acc);
