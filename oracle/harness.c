@@ -25,8 +25,27 @@
 #include "pumas_gpu.h"
 
 #include <pthread.h>
+#include <setjmp.h>
+#include <signal.h>
 #include <stdint.h>
 #include <time.h>
+
+/*
+ * Histories on which the reference aborts (an assert in
+ * del_randomise_target, src/pumas.c:6264, reached in backward transport
+ * when the selection pass finds no target) are caught and reported with
+ * this return code instead of ending the process.
+ */
+#define REF_ABORTED (-1)
+
+static __thread sigjmp_buf * ref_jump = NULL;
+
+static void ref_abort_handler(int sig)
+{
+        if (ref_jump != NULL) siglongjmp(*ref_jump, 1);
+        signal(sig, SIG_DFL);
+        raise(sig);
+}
 
 /* ------------------------------------------------------------------------
  * Philox4x32-10 (Salmon et al., "Parallel random numbers: as easy as
@@ -285,8 +304,20 @@ static void * ref_worker(void * arg)
                 ctx->lifetime = 0.;
                 enum pumas_event event = PUMAS_EVENT_NONE;
                 struct pumas_medium * media[2] = { NULL, NULL };
-                const enum pumas_return rc = pumas_context_transport(
-                    context, job->states + i, &event, media);
+                struct pumas_state backup = job->states[i];
+                sigjmp_buf jump;
+                int rc;
+                if (sigsetjmp(jump, 1) == 0) {
+                        ref_jump = &jump;
+                        rc = pumas_context_transport(
+                            context, job->states + i, &event, media);
+                        ref_jump = NULL;
+                } else {
+                        ref_jump = NULL;
+                        rc = REF_ABORTED;
+                        job->states[i] = backup;
+                        media[0] = media[1] = NULL;
+                }
                 if (job->events) job->events[i] = event;
                 if (job->media) {
                         int k;
@@ -317,6 +348,11 @@ double ref_transport(const void * physics,
     int n_threads, uint64_t * total_steps)
 {
         if (n_threads < 1) n_threads = 1;
+        struct sigaction sa, previous;
+        memset(&sa, 0x0, sizeof(sa));
+        sa.sa_handler = &ref_abort_handler;
+        sigemptyset(&sa.sa_mask);
+        sigaction(SIGABRT, &sa, &previous);
         struct ref_job * jobs = calloc(n_threads, sizeof(*jobs));
         pthread_t * threads = calloc(n_threads, sizeof(*threads));
         struct timespec t0, t1;
@@ -350,6 +386,7 @@ double ref_transport(const void * physics,
         }
         clock_gettime(CLOCK_MONOTONIC, &t1);
         if (total_steps) *total_steps = total;
+        sigaction(SIGABRT, &previous, NULL);
         free(jobs);
         free(threads);
         return (t1.tv_sec - t0.tv_sec) + 1E-09 * (t1.tv_nsec - t0.tv_nsec);

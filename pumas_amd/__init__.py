@@ -22,7 +22,7 @@ from ._abi import (Event, GeometryKind, Mode, Precision, Return, Step,
                    STATE_DTYPE)
 
 __all__ = [
-    "Context", "Event", "Geometry", "GeometryKind", "Mode", "Physics",
+    "Context", "Event", "Geometry", "GeometryDescription", "GeometryKind", "Mode", "Physics",
     "Precision", "PumasError", "Return", "STATE_DTYPE", "Step", "library",
     "new_states", "physics_path", "transport",
 ]

@@ -137,6 +137,8 @@ struct Rng {
         uint32_t h0, h1;
         uint32_t stream;
         uint32_t count;
+        /* Second word of the last block, for the following odd draw. */
+        uint32_t next_lo, next_hi;
 
         PG_FN void init(uint64_t seed, uint64_t history, uint32_t stream_,
             uint32_t count_)
@@ -147,18 +149,28 @@ struct Rng {
                 h1 = (uint32_t)(history >> 32);
                 stream = stream_;
                 count = count_;
+                if (count & 1u) {
+                        const Philox4 r =
+                            philox4x32_10(count >> 1, stream, h0, h1, k0, k1);
+                        next_lo = r.x[2];
+                        next_hi = r.x[3];
+                }
+        }
+
+        /* 64 random bits of draw `count`; a block serves two draws. */
+        PG_FN uint64_t bits()
+        {
+                const uint32_t i = count++;
+                if (i & 1u) return ((uint64_t)next_hi << 32) | next_lo;
+                const Philox4 r = philox4x32_10(i >> 1, stream, h0, h1, k0, k1);
+                next_lo = r.x[2];
+                next_hi = r.x[3];
+                return ((uint64_t)r.x[1] << 32) | r.x[0];
         }
 
         PG_FN double uniform()
         {
-                const uint32_t i = count++;
-                const Philox4 r =
-                    philox4x32_10(i >> 1, stream, h0, h1, k0, k1);
-                uint64_t w;
-                if (i & 1u)
-                        w = ((uint64_t)r.x[3] << 32) | r.x[2];
-                else
-                        w = ((uint64_t)r.x[1] << 32) | r.x[0];
+                const uint64_t w = bits();
                 return ((double)(w >> 11) + 0.5) * (1.0 / 9007199254740992.0);
         }
 };

@@ -182,3 +182,168 @@ def slabs_backward_context(energy_loss=Mode.STRAGGLED):
     return Context(energy_loss=energy_loss, decay=Mode.WEIGHTED,
                    direction=Mode.BACKWARD, scattering=Mode.MIXED,
                    event=Event.NONE)
+
+
+# ---------------------------------------------------------------------------
+# Terrain: heightmap of seeded fractal value noise, backward muography.
+# ---------------------------------------------------------------------------
+def value_noise_2d(nx, ny, seed, octaves=6, base_cells=4, gain=0.5):
+    """Fractal (fBm) value noise on an [ny, nx] node grid, in [0, 1]: sum of
+    octaves of bilinearly interpolated random lattices drawn from the
+    Philox stream (stream 2, history = octave)."""
+    out = np.zeros((ny, nx))
+    amp, total = 1.0, 0.0
+    for o in range(octaves):
+        cells = base_cells * 2 ** o
+        ids = np.arange((cells + 1) ** 2, dtype=np.uint64) + np.uint64(
+            o << 32)
+        lattice = uniform(seed, 2, ids, 0).reshape(cells + 1, cells + 1)
+        fx = np.linspace(0, cells, nx, endpoint=True)
+        fy = np.linspace(0, cells, ny, endpoint=True)
+        ix = np.minimum(fx.astype(int), cells - 1)
+        iy = np.minimum(fy.astype(int), cells - 1)
+        hx = (fx - ix)[None, :]
+        hy = (fy - iy)[:, None]
+        l00 = lattice[iy[:, None], ix[None, :]]
+        l10 = lattice[iy[:, None], ix[None, :] + 1]
+        l01 = lattice[iy[:, None] + 1, ix[None, :]]
+        l11 = lattice[iy[:, None] + 1, ix[None, :] + 1]
+        out += amp * ((l00 * (1 - hx) + l10 * hx) * (1 - hy) +
+                      (l01 * (1 - hx) + l11 * hx) * hy)
+        total += amp
+        amp *= gain
+    out /= total
+    return (out - out.min()) / (out.max() - out.min())
+
+
+TERRAIN_Z0 = -500.0
+TERRAIN_ZTOP = 1000.0
+TERRAIN_DETECTOR_Z = -100.0
+
+
+def terrain_geometry(n=1024, spacing=5.0, relief=800.0, seed=0,
+                     rho_min=2000.0, rho_max=2900.0):
+    """Heightmap of n x n nodes at `spacing` m with heights in [0, relief] m
+    and one standard rock column per cell with a density uniform in
+    [rho_min, rho_max] kg/m^3; air above, to TERRAIN_ZTOP."""
+    h = relief * value_noise_2d(n, n, seed)
+    ids = np.arange((n - 1) ** 2, dtype=np.uint64)
+    rho = rho_min + (rho_max - rho_min) * uniform(seed, 3, ids, 0)
+    half = 0.5 * (n - 1) * spacing
+    return GeometryDescription.terrain(
+        h.astype(np.float32), rho.reshape(n - 1, n - 1).astype(np.float32),
+        x0=-half, y0=-half, dx=spacing, dy=spacing, z0=TERRAIN_Z0,
+        z_top=TERRAIN_ZTOP, air=(AIR, 0.0), ground_material=ROCK)
+
+
+def terrain_states(n, extent, seed=0, first=0, kmin=1e-2, kmax=1e2,
+                   cmin=0.5, cmax=1.0):
+    """Final states at detectors under the terrain: positions uniform in
+    the central half of the map at TERRAIN_DETECTOR_Z, momentum going down
+    in a cone cos(zenith) in [cmin, cmax], K log-uniform, weight 1/pdf."""
+    ids = _ids(n, first)
+    u = [uniform(seed, 0, ids, k) for k in range(5)]
+    rk = math.log(kmax / kmin)
+    s = np.zeros(n, STATE_DTYPE)
+    s["charge"] = -1.0
+    s["energy"] = kmin * np.exp(rk * u[0])
+    s["weight"] = s["energy"] * rk
+    ct = cmin + (cmax - cmin) * u[1]
+    st = np.sqrt(1.0 - ct * ct)
+    phi = 2.0 * math.pi * u[2]
+    s["direction"][:, 0] = st * np.cos(phi)
+    s["direction"][:, 1] = st * np.sin(phi)
+    s["direction"][:, 2] = -ct
+    s["position"][:, 0] = 0.5 * extent * (u[3] - 0.5)
+    s["position"][:, 1] = 0.5 * extent * (u[4] - 0.5)
+    s["position"][:, 2] = TERRAIN_DETECTOR_Z
+    return s
+
+
+def terrain_context():
+    return Context(energy_loss=Mode.STRAGGLED, decay=Mode.WEIGHTED,
+                   direction=Mode.BACKWARD, scattering=Mode.MIXED,
+                   event=Event.NONE)
+
+
+# ---------------------------------------------------------------------------
+# Voxels: rock / water / air / iron by 3-D noise thresholds.
+# ---------------------------------------------------------------------------
+def value_noise_3d(n, seed, octaves=4, base_cells=2):
+    out = np.zeros((n, n, n))
+    amp, total = 1.0, 0.0
+    for o in range(octaves):
+        cells = base_cells * 2 ** o
+        ids = np.arange((cells + 1) ** 3, dtype=np.uint64) + np.uint64(
+            o << 32)
+        lat = uniform(seed, 4, ids, 0).reshape((cells + 1,) * 3)
+        f = np.linspace(0, cells, n, endpoint=True)
+        i = np.minimum(f.astype(int), cells - 1)
+        h = f - i
+        hz, hy, hx = h[:, None, None], h[None, :, None], h[None, None, :]
+        iz, iy, ix = i[:, None, None], i[None, :, None], i[None, None, :]
+        acc = 0.0
+        for dz in (0, 1):
+            for dy in (0, 1):
+                for dx in (0, 1):
+                    w = ((hz if dz else 1 - hz) * (hy if dy else 1 - hy) *
+                         (hx if dx else 1 - hx))
+                    acc = acc + w * lat[iz + dz, iy + dy, ix + dx]
+        out += amp * acc
+        total += amp
+        amp *= 0.5
+    return out / total
+
+
+VOXEL_MATERIALS = ((ROCK, 0.0), (WATER, 0.0), (AIR, 0.0), (IRON, 0.0))
+VOXEL_DEFAULT_DENSITY = (2650.0, 1000.0, 1.205, 7874.0)
+
+
+def voxel_geometry(n=512, size=2.0, seed=0, sigma=0.1):
+    """n^3 voxels of `size` m; material classes by thresholds on 3-D value
+    noise (air < 0.35 < water < 0.45 < rock < 0.7 < iron) and a lognormal
+    density scale of mean one and log-sigma `sigma` per voxel."""
+    noise = value_noise_3d(n, seed)
+    label = np.full(noise.shape, 0, np.uint8)      # rock
+    label[noise < 0.45] = 1                        # water
+    label[noise < 0.35] = 2                        # air
+    label[noise >= 0.7] = 3                        # iron
+    ids = np.arange(n ** 3, dtype=np.uint64)
+    u1 = uniform(seed, 5, ids, 0)
+    u2 = uniform(seed, 5, ids, 1)
+    g = np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * math.pi * u2)
+    scale = np.exp(sigma * g - 0.5 * sigma * sigma)
+    rho = np.asarray(VOXEL_DEFAULT_DENSITY)[label] * scale.reshape(label.shape)
+    half = 0.5 * n * size
+    return GeometryDescription.voxels(
+        label, rho.astype(np.float32), VOXEL_MATERIALS, x0=-half, y0=-half,
+        z0=-half, dx=size, dy=size, dz=size)
+
+
+def voxel_states(n, extent, seed=0, first=0, kmin=1e-1, kmax=1e3,
+                 cmin=0.5, cmax=1.0):
+    """Final states at detectors near the bottom of the voxel cube, momentum
+    going down in a cone, K log-uniform in [kmin, kmax] GeV."""
+    ids = _ids(n, first)
+    u = [uniform(seed, 0, ids, k) for k in range(5)]
+    rk = math.log(kmax / kmin)
+    s = np.zeros(n, STATE_DTYPE)
+    s["charge"] = -1.0
+    s["energy"] = kmin * np.exp(rk * u[0])
+    s["weight"] = s["energy"] * rk
+    ct = cmin + (cmax - cmin) * u[1]
+    st = np.sqrt(1.0 - ct * ct)
+    phi = 2.0 * math.pi * u[2]
+    s["direction"][:, 0] = st * np.cos(phi)
+    s["direction"][:, 1] = st * np.sin(phi)
+    s["direction"][:, 2] = -ct
+    s["position"][:, 0] = 0.25 * extent * (u[3] - 0.5)
+    s["position"][:, 1] = 0.25 * extent * (u[4] - 0.5)
+    s["position"][:, 2] = -0.45 * extent
+    return s
+
+
+def voxel_context():
+    return Context(energy_loss=Mode.STRAGGLED, decay=Mode.WEIGHTED,
+                   direction=Mode.BACKWARD, scattering=Mode.MIXED,
+                   event=Event.NONE)

@@ -50,3 +50,13 @@ def ref_standard_aligned():
     p = orc.RefPhysics("standard_c005", aligned=True)
     yield p
     p.close()
+
+
+@pytest.fixture(scope="session")
+def ref_iron_aligned():
+    import oracle as orc
+    if not orc.available(aligned=True):
+        pytest.skip("aligned oracle library not built (make -C oracle)")
+    p = orc.RefPhysics("iron_c001", aligned=True)
+    yield p
+    p.close()

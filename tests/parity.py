@@ -28,29 +28,40 @@ def weight_errors(a, b):
 
 
 def assert_parity(gpu, ref, rtol, rtol_strict=1e-12, frac_strict=0.97,
-                  counters=True, weight_rtol=None):
+                  counters=True, weight_rtol=None, max_flips=0.0):
     """Events, media, step and draw counts identical; every history within
     `rtol`, and at least `frac_strict` of them within `rtol_strict`. When
-    `weight_rtol` is given the weight is checked against it separately."""
+    `weight_rtol` is given the weight is checked against it separately.
+    Up to a fraction `max_flips` of the histories may branch differently
+    (different counters, or values beyond the tolerances); they are
+    reported and left out of the strict fraction."""
+    final = ref["final"] if isinstance(ref, dict) else ref.states
+    flipped = np.zeros(len(final), bool)
     if counters:
         for k in ("events", "media", "steps", "draws", "rc"):
             g, r = getattr(gpu, k), ref[k] if isinstance(ref, dict) else \
                 getattr(ref, k)
-            bad = np.nonzero(np.atleast_2d((g != r).T).any(axis=0))[0]
-            assert len(bad) == 0, f"{k} differ for histories {bad[:10]}"
-    final = ref["final"] if isinstance(ref, dict) else ref.states
+            flipped |= np.atleast_2d((g != r).T).any(axis=0)
     fields = FIELDS if weight_rtol is None else FIELDS[:-1]
     err = state_errors(gpu.states, final, fields)
+    werr = weight_errors(gpu.states, final)
+    outliers = err > rtol
     if weight_rtol is not None:
-        werr = weight_errors(gpu.states, final)
-        assert werr.max() <= weight_rtol, f"weight error {werr.max()}"
-    assert (gpu.states["decayed"] == final["decayed"]).all()
-    worst = int(np.argmax(err))
-    detail = {f: float(state_errors(gpu.states[worst:worst + 1],
-                                    final[worst:worst + 1], (f,))[0])
-              for f in FIELDS}
-    assert err.max() <= rtol, \
-        f"history {worst}: relative error {err[worst]} {detail}"
-    strict = float(np.mean(err <= rtol_strict))
+        outliers |= werr > weight_rtol
+    outliers |= gpu.states["decayed"] != final["decayed"]
+    bad = np.nonzero(flipped | outliers)[0]
+    if len(bad):
+        print(f"parity: {len(bad)} of {len(final)} histories branch "
+              f"differently (budget {max_flips:.2%})")
+    if len(bad) > max_flips * len(final):
+        worst = int(bad[0])
+        detail = {f: float(state_errors(gpu.states[worst:worst + 1],
+                                        final[worst:worst + 1], (f,))[0])
+                  for f in FIELDS}
+        raise AssertionError(
+            f"{len(bad)} histories differ (first {bad[:10]}); history "
+            f"{worst}: flipped={bool(flipped[worst])} errors {detail}")
+    keep = ~(flipped | outliers)
+    strict = float(np.mean(err[keep] <= rtol_strict)) if keep.any() else 1.0
     assert strict >= frac_strict, f"only {strict:.4f} within {rtol_strict}"
     return err
