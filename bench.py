@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target duration of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64",
+                    help="fp64: reference arithmetic; fp32: FP32-bulk")
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     return ap.parse_args()
@@ -107,11 +109,14 @@ def main():
     work = torch.empty_like(pristine)
     stream = torch.cuda.current_stream().cuda_stream
 
+    precision = pa.Precision.FP64 if args.precision == "fp64" else \
+        pa.Precision.FP32_BULK
+
     def one_pass(k):
         work.copy_(pristine)
         return pa.transport_device(
             phys, ctx, geom, work.data_ptr(), n, seed=0,
-            first_history=k * n, stream=stream,
+            first_history=k * n, stream=stream, precision=precision,
             block_size=args.block_size, blocks_per_cu=args.blocks_per_cu)
 
     for k in range(args.warmup):
@@ -159,15 +164,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {
             "workload": "forward straggled + mixed scattering, 3 slabs "
                         "(100 m air / 500 m standard rock / 200 m water), "
                         "K0 log-uniform [10, 1e4] GeV, cos(zenith) in "
-                        "[0.5, 1], FP64",
+                        "[0.5, 1]",
             "histories": n,
-            "precision": "fp64",
+            "precision": args.precision,
         },
         "roofline": {
             "bound": "hbm",

@@ -1,0 +1,191 @@
+/*
+ * FP32-bulk table interpolation: single precision mirrors of the CEL, DEL
+ * and scattering tables and the lookups the FP32-bulk engine makes on them
+ * every step. Layouts are those of the FP64 tables (src/pumas.c:4260-4752);
+ * the arithmetic is the same as in engine_common.h, in float.
+ */
+#ifndef PUMAS_ENGINE_BULK_H
+#define PUMAS_ENGINE_BULK_H
+
+#include "engine_common.h"
+
+namespace pg {
+
+struct FloatTables {
+        int n_energies, n_materials;
+        float mass;
+        const float *K, *X, *X_dK, *dE, *dE_dK, *Omega, *Omega_dK;
+        const float *CS, *CS_dK, *Mu0, *Mu0_dK, *Ms1, *Ms1_dK, *Lb;
+        const float *a_max, *b_max;
+};
+
+/* Bracketing rows memory for float grids (see Cache). */
+struct FloatCache {
+        int k[2];
+        int x[2];
+
+        PG_FN void reset(int n_energies)
+        {
+                k[0] = k[1] = x[0] = x[1] = n_energies - 2;
+        }
+};
+
+PG_FN int findex(const float * table, int n, float value, int * last)
+{
+        if ((value >= table[last[0]]) && (value < table[last[0] + 1]))
+                return last[0];
+        if ((value >= table[last[1]]) && (value < table[last[1] + 1]))
+                return last[1];
+        const int imax = n - 1;
+        if (value < table[0]) return -1;
+        if (value >= table[imax]) return imax;
+        int i1 = 0, i2 = imax;
+        do {
+                const int i3 = (i1 + i2) / 2;
+                if (value >= table[i3])
+                        i1 = i3;
+                else
+                        i2 = i3;
+        } while (i2 - i1 >= 2);
+        if (i1 != last[0])
+                last[0] = i1;
+        else if (i1 != last[1])
+                last[1] = i1;
+        return i1;
+}
+
+PG_FN float fpchip(float t, float p0, float p1, float m0, float m1)
+{
+        const float c2 = -3.f * (p0 - p1) - 2.f * m0 - m1;
+        const float c3 = 2.f * (p0 - p1) + m0 + m1;
+        return p0 + t * (m0 + t * (c2 + t * c3));
+}
+
+PG_FN float finterpolate(const FloatTables & f, FloatCache & ch,
+    const float * tx, const float * ty, const float * tm, float x)
+{
+        const int i1 = findex(tx, f.n_energies, x, (tx == f.K) ? ch.k : ch.x);
+        const int i2 = i1 + 1;
+        const float dX = tx[i2] - tx[i1];
+        const float t = (x - tx[i1]) / dX;
+        const float m1 = tm[i1] * dX;
+        const float m2 = (i2 > 1) ? tm[i2] * dX : m1;
+        return fpchip(t, ty[i1], ty[i2], m1, m2);
+}
+
+PG_FN const float * fcol2(
+    const FloatTables & f, const float * t, int scheme, int material)
+{
+        return t + (scheme2(scheme) * f.n_materials + material) * f.n_energies;
+}
+
+PG_FN const float * fcol1(const FloatTables & f, const float * t, int material)
+{
+        return t + material * f.n_energies;
+}
+
+/* CSDA range, as cel_grammage. */
+PG_FN float fcel_grammage(const FloatTables & f, FloatCache & ch, int scheme,
+    int material, float kinetic)
+{
+        const int imax = f.n_energies - 1;
+        if (kinetic < f.K[0]) return 0.f;
+        const float * X = fcol2(f, f.X, scheme, material);
+        if (kinetic >= f.K[imax]) {
+                const float a = f.a_max[material];
+                const float b = f.b_max[scheme2(scheme) * f.n_materials +
+                    material];
+                const float K1 = a / b + f.mass;
+                return X[imax] +
+                    1.f / b * logf((kinetic + K1) / (f.K[imax] + K1));
+        }
+        return finterpolate(f, ch, f.K, X, fcol2(f, f.X_dK, scheme, material),
+            kinetic);
+}
+
+/* Stopping power, as cel_energy_loss. */
+PG_FN float fcel_energy_loss(const FloatTables & f, FloatCache & ch,
+    int scheme, int material, float kinetic)
+{
+        const int imax = f.n_energies - 1;
+        if (kinetic < f.K[0]) return 0.f;
+        if (kinetic >= f.K[imax])
+                return f.a_max[material] +
+                    f.b_max[scheme2(scheme) * f.n_materials + material] *
+                    (kinetic + f.mass);
+        return finterpolate(f, ch, f.K, fcol2(f, f.dE, scheme, material),
+            fcol2(f, f.dE_dK, scheme, material), kinetic);
+}
+
+/* Straggling, as cel_straggling. */
+PG_FN float fcel_straggling(
+    const FloatTables & f, FloatCache & ch, int material, float kinetic)
+{
+        const int imax = f.n_energies - 1;
+        if (kinetic < f.K[0]) return 0.f;
+        const float * Om = fcol1(f, f.Omega, material);
+        if (kinetic >= f.K[imax]) {
+                const float r = (kinetic + f.mass) / (f.K[imax] + f.mass);
+                return Om[imax] * r * r;
+        }
+        return finterpolate(
+            f, ch, f.K, Om, fcol1(f, f.Omega_dK, material), kinetic);
+}
+
+/* DEL cross section, as del_cross_section. */
+PG_FN float fdel_cross_section(
+    const FloatTables & f, FloatCache & ch, int material, float kinetic)
+{
+        const int imax = f.n_energies - 1;
+        if (kinetic < f.K[0]) return 0.f;
+        const float * CS = fcol1(f, f.CS, material);
+        if (kinetic >= f.K[imax]) return CS[imax];
+        return finterpolate(
+            f, ch, f.K, CS, fcol1(f, f.CS_dK, material), kinetic);
+}
+
+/* Multiple scattering parameters, as table_get_msc. */
+PG_FN void ftable_get_msc(const FloatTables & f, FloatCache & ch, int scheme,
+    int material, float kinetic, float * mu0, float * invlb1)
+{
+        const int imax = f.n_energies - 1;
+        const int s = scheme2(scheme) + 1;
+        const float * Mu0 = fcol1(f, f.Mu0, material);
+        const float * Ms1 = f.Ms1 + (s * f.n_materials + material) *
+            f.n_energies;
+        if (kinetic < f.K[1]) {
+                *mu0 = Mu0[1];
+                *invlb1 = Ms1[1] * sqrtf(f.K[1] / kinetic);
+        } else if (kinetic >= f.K[imax]) {
+                *mu0 = Mu0[imax];
+                *invlb1 = Ms1[imax] * f.K[imax] / kinetic;
+        } else {
+                const int i1 = findex(f.K, f.n_energies, kinetic, ch.k);
+                const float * m0 = fcol1(f, f.Mu0_dK, material) + i1;
+                const float * m1 = f.Ms1_dK +
+                    (s * f.n_materials + material) * f.n_energies + i1;
+                const float dx = f.K[i1 + 1] - f.K[i1];
+                const float t = (kinetic - f.K[i1]) / dx;
+                *mu0 = fpchip(t, Mu0[i1], Mu0[i1 + 1], m0[0] * dx, m0[1] * dx);
+                *invlb1 = fpchip(
+                    t, Ms1[i1], Ms1[i1 + 1], m1[0] * dx, m1[1] * dx);
+        }
+}
+
+/* Hard elastic path, as coulomb_ehs_length. */
+PG_FN float fcoulomb_ehs_length(
+    const FloatTables & f, FloatCache & ch, int material, float kinetic)
+{
+        const float p2 = kinetic * (kinetic + 2.f * f.mass);
+        const int imax = f.n_energies - 1;
+        const float * Lb = fcol1(f, f.Lb, material);
+        if (kinetic < f.K[1]) return Lb[1] / p2;
+        if (kinetic >= f.K[imax]) return Lb[imax] / p2;
+        const int i1 = findex(f.K, f.n_energies, kinetic, ch.k);
+        const float h = (kinetic - f.K[i1]) / (f.K[i1 + 1] - f.K[i1]);
+        return (Lb[i1] + h * (Lb[i1 + 1] - Lb[i1])) / p2;
+}
+
+} /* namespace pg */
+
+#endif

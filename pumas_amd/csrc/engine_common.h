@@ -173,6 +173,10 @@ struct Rng {
                 const uint64_t w = bits();
                 return ((double)(w >> 11) + 0.5) * (1.0 / 9007199254740992.0);
         }
+
+        /* The same draw rounded to single precision (FP32-bulk engine),
+         * so that both engines consume one random sequence. */
+        PG_FN float uniformf() { return (float)uniform(); }
 };
 
 /* ------------------------------------------------------------------------

@@ -10,6 +10,7 @@
 #ifndef PUMAS_ENGINE_TRANSPORT_H
 #define PUMAS_ENGINE_TRANSPORT_H
 
+#include "engine_bulk.h"
 #include "engine_interactions.h"
 #include "geometry.h"
 
@@ -29,6 +30,7 @@ struct Track {
         double randn_next;
         /* Table row memory, random stream and step counter. */
         Cache ix;
+        FloatCache fx;
         Rng rng;
         uint32_t steps;
         int max_steps;
@@ -54,11 +56,96 @@ struct Walker {
         double ti, wi, Xi, Xf, dei, grammage_max;
 };
 
-struct Engine {
+/*
+ * The transport engine. With F32 false every operation is the reference's
+ * FP64 arithmetic. With F32 true (FP32-bulk) the per-step table lookups,
+ * the straggling, scattering and random number sampling run in single
+ * precision; the state (energy, distance, grammage, time, weight, position,
+ * direction), the range to energy inversion, the event weights and the
+ * geometry stay in double precision.
+ */
+template <bool F32> struct EngineT {
         const PhysicsView & p;
         const ModelView & mv;
         const Config & c;
         const pg_geometry & g;
+        const FloatTables & ft;
+
+        /* Per-step table lookups, in the bulk precision. */
+        PG_FN double bulk_grammage(Track & t, int scheme, int material,
+            double kinetic) const
+        {
+                if constexpr (F32)
+                        return fcel_grammage(
+                            ft, t.fx, scheme, material, (float)kinetic);
+                else
+                        return cel_grammage(p, t.ix, scheme, material, kinetic);
+        }
+
+        PG_FN double bulk_energy_loss(Track & t, int scheme, int material,
+            double kinetic) const
+        {
+                if constexpr (F32)
+                        return fcel_energy_loss(
+                            ft, t.fx, scheme, material, (float)kinetic);
+                else
+                        return cel_energy_loss(
+                            p, t.ix, scheme, material, kinetic);
+        }
+
+        PG_FN double bulk_straggling(Track & t, int material,
+            double kinetic) const
+        {
+                if constexpr (F32)
+                        return fcel_straggling(
+                            ft, t.fx, material, (float)kinetic);
+                else
+                        return cel_straggling(p, t.ix, material, kinetic);
+        }
+
+        PG_FN double bulk_del_cross_section(Track & t, int material,
+            double kinetic) const
+        {
+                if constexpr (F32)
+                        return fdel_cross_section(
+                            ft, t.fx, material, (float)kinetic);
+                else
+                        return del_cross_section(p, t.ix, material, kinetic);
+        }
+
+        PG_FN double bulk_ehs_length(Track & t, int material,
+            double kinetic) const
+        {
+                if constexpr (F32)
+                        return fcoulomb_ehs_length(
+                            ft, t.fx, material, (float)kinetic);
+                else
+                        return coulomb_ehs_length(p, t.ix, material, kinetic);
+        }
+
+        PG_FN void bulk_msc(Track & t, int scheme, int material,
+            double kinetic, double * mu0, double * invlb1) const
+        {
+                if constexpr (F32) {
+                        float m, l;
+                        ftable_get_msc(ft, t.fx, scheme, material,
+                            (float)kinetic, &m, &l);
+                        *mu0 = m;
+                        *invlb1 = l;
+                } else {
+                        table_get_msc(
+                            p, t.ix, scheme, material, kinetic, mu0, invlb1);
+                }
+        }
+
+        /* -ln(u) for an exponential path length or angle. */
+        PG_FN double bulk_neglog_uniform(Track & t) const
+        {
+                if constexpr (F32)
+                        return -(double)logf(t.rng.uniformf());
+                else
+                        return -log(t.rng.uniform());
+        }
 
         PG_FN double sgn() const
         {
@@ -95,6 +182,15 @@ struct Engine {
         {
                 t.randn_done = !t.randn_done;
                 if (!t.randn_done) return t.randn_next;
+                if constexpr (F32) {
+                        const float r = sqrtf(-2.f * logf(t.rng.uniformf()));
+                        const float phi =
+                            2.f * (float)PI * t.rng.uniformf();
+                        float s, cp;
+                        sincosf(phi, &s, &cp);
+                        t.randn_next = r * cp;
+                        return r * s;
+                }
                 const double r = sqrt(-2. * log(t.rng.uniform()));
                 const double phi = 2. * PI * t.rng.uniform();
                 const double cp = cos(phi);
@@ -149,9 +245,19 @@ struct Engine {
                 const double u1y = u0z * direction[0] - u0x * direction[2];
                 const double u1z = u0x * direction[1] - u0y * direction[0];
 
-                const double phi = PI * (1. - 2. * t.rng.uniform());
-                const double cp = cos(phi);
-                const double sp = sin(phi);
+                double cp, sp;
+                if constexpr (F32) {
+                        const float phi =
+                            (float)PI * (1.f - 2.f * t.rng.uniformf());
+                        float sf, cf;
+                        sincosf(phi, &sf, &cf);
+                        cp = cf;
+                        sp = sf;
+                } else {
+                        const double phi = PI * (1. - 2. * t.rng.uniform());
+                        cp = cos(phi);
+                        sp = sin(phi);
+                }
                 direction[0] =
                     cos_theta * direction[0] + st * (cp * u0x + sp * u1x);
                 direction[1] =
@@ -403,14 +509,15 @@ struct Engine {
                 const double dk0 = fabs(t.energy - k1);
                 if ((k1 > 0.) && (dk0 > 0.)) {
                         const double Omega0 =
-                            cel_straggling(p, t.ix, material, t.energy);
-                        const double Omega1 = cel_straggling(p, t.ix, material, k1);
+                            bulk_straggling(t, material, t.energy);
+                        const double Omega1 = bulk_straggling(t, material, k1);
                         double dk12 = 0.5 * dX * (Omega0 + Omega1);
                         const double tmp = dX / Xtot;
                         if (tmp > 5E-02) {
                                 const double de1 =
-                                    cel_energy_loss(p, t.ix, scheme, material, k1);
-                                const double de0 = cel_energy_loss(p, t.ix, scheme, material, t.energy);
+                                    bulk_energy_loss(t, scheme, material, k1);
+                                const double de0 = bulk_energy_loss(
+                                    t, scheme, material, t.energy);
                                 dk12 *= 1. + s * dX / dk0 * (de1 - de0);
                         }
                         const double dk1 = sqrt(dk12);
@@ -464,7 +571,7 @@ struct Engine {
                 const int tmp_scheme =
                     (scheme == MODE_DISABLED) ? MODE_CSDA : scheme;
                 const double Xtot =
-                    cel_grammage(p, t.ix, tmp_scheme, material, t.energy);
+                    bulk_grammage(t, tmp_scheme, material, t.energy);
 
                 double step_loc;
                 double invlb1 = 0.;
@@ -482,7 +589,7 @@ struct Engine {
                         if (c.scattering == MODE_MIXED) {
                                 if (t.step_first != 0) {
                                         double mu0;
-                                        table_get_msc(p, t.ix, scheme, material,
+                                        bulk_msc(t, scheme, material,
                                             t.energy, &mu0, &invlb1);
                                         invlb1 *= density;
                                 } else
@@ -614,7 +721,7 @@ struct Engine {
                                 }
                         } else if (dX < EPSILON_X * Xtot) {
                                 k1 -= sg *
-                                    cel_energy_loss(p, t.ix, scheme, material, k1) *
+                                    bulk_energy_loss(t, scheme, material, k1) *
                                     dX;
                                 if ((c.event & EVENT_LIMIT_ENERGY) &&
                                     (c.limit_energy > 0.)) {
@@ -670,20 +777,20 @@ struct Engine {
                         /* In-step inelastic DEL. */
                         do {
                                 double xs_del =
-                                    del_cross_section(p, t.ix, material, t.energy);
+                                    bulk_del_cross_section(t, material, t.energy);
                                 const double tmp_del =
-                                    del_cross_section(p, t.ix, material, k1);
+                                    bulk_del_cross_section(t, material, k1);
                                 if (tmp_del > xs_del) xs_del = tmp_del;
                                 if (xs_del <= 0.) break;
                                 const double X_del =
-                                    -log(t.rng.uniform()) / xs_del;
+                                    bulk_neglog_uniform(t) / xs_del;
                                 if (X_del >= Xmax) break;
                                 const double k_h = cel_kinetic_energy(p, t.ix, scheme, material, Xtot - sg * X_del);
                                 const double k_del = t.energy -
                                     sg * ratio * fabs(t.energy - k_h);
                                 if (k_del <= 0.) break;
                                 const double r =
-                                    del_cross_section(p, t.ix, material, k_del) /
+                                    bulk_del_cross_section(t, material, k_del) /
                                     xs_del;
                                 if (t.rng.uniform() > r) break;
                                 k_x = k_del;
@@ -703,17 +810,17 @@ struct Engine {
                                         kmin = kmax, kmax = tmp;
                                 }
                                 const double lb_ehs =
-                                    coulomb_ehs_length(p, t.ix, material, kmin);
+                                    bulk_ehs_length(t, material, kmin);
                                 if (lb_ehs <= 0.) break;
                                 const double X_ehs =
-                                    -log(t.rng.uniform()) * lb_ehs;
+                                    bulk_neglog_uniform(t) * lb_ehs;
                                 if (X_ehs >= Xmax) break;
                                 const double k_h = cel_kinetic_energy(p, t.ix, scheme, material, Xtot - sg * X_ehs);
                                 const double k_ehs = t.energy -
                                     sg * ratio * fabs(t.energy - k_h);
                                 if (k_ehs <= 0.) break;
                                 const double r =
-                                    coulomb_ehs_length(p, t.ix, material, k_ehs) /
+                                    bulk_ehs_length(t, material, k_ehs) /
                                     lb_ehs;
                                 if ((r <= 0.) || (t.rng.uniform() > 1. / r))
                                         break;
@@ -866,7 +973,7 @@ struct Engine {
                         if (t.energy <= 0.) {
                                 t.step_invlb1 = invlb1;
                         } else {
-                                table_get_msc(p, t.ix, scheme, material, t.energy,
+                                bulk_msc(t, scheme, material, t.energy,
                                     &mu0, &invlb1_);
                                 t.step_invlb1 = locals_density * invlb1_;
                         }
@@ -877,7 +984,7 @@ struct Engine {
                         if (ilb1 > 1.) ilb1 = 1.;
                         double mu;
                         do {
-                                mu = -ilb1 * log(t.rng.uniform());
+                                mu = ilb1 * bulk_neglog_uniform(t);
                         } while (mu > 1.);
                         rotate(t, mu);
                 }
@@ -1259,32 +1366,47 @@ struct Engine {
                 }
         }
 
-        /* Coulomb scattering data of component i of a material, as set up
-         * per element by transport_do_ehs (src/pumas.c:5786-5815). */
-        PG_FN void coulomb_setup(CoulombData & d, int material, int i,
-            double kinetic, double mu0) const
+        /* Scattering parameters of component i of a material: CM frame,
+         * spin factor and screening, as set up per element by
+         * transport_do_ehs (src/pumas.c:5786-5803). */
+        PG_FN void coulomb_parameters(CoulombData & d, int material, int i,
+            double kinetic, double * kinetic0) const
         {
                 const int ic = component0(material) + i;
                 const int iel = p.component_element[ic];
                 const double Z = p.element_Z[iel];
                 const double A = p.element_A[iel];
-                double kinetic0;
                 coulomb_frame_parameters(
-                    Z, A, p.mass, kinetic, &kinetic0, d.fCM);
+                    Z, A, p.mass, kinetic, kinetic0, d.fCM);
                 d.fspin = coulomb_spin_factor(p.mass, kinetic);
-                coulomb_screening_parameters(p, iel, kinetic, kinetic0,
+                coulomb_screening_parameters(p, iel, kinetic, *kinetic0,
                     &d.n_parameters, d.amplitude, d.screening);
+        }
+
+        /* Restricted hard cross section of component i, the remainder of
+         * the per element set up (src/pumas.c:5804-5815). */
+        PG_FN double coulomb_cs_hard(int material, int i, double kinetic,
+            double mu0) const
+        {
+                CoulombData d;
+                double kinetic0;
+                coulomb_parameters(d, material, i, kinetic, &kinetic0);
                 coulomb_pole_reduction(d);
+                const int ic = component0(material) + i;
+                const int iel = p.component_element[ic];
                 d.normalisation = p.component_fraction[ic] /
-                    coulomb_normalisation(Z, A, p.mass, kinetic, kinetic0);
-                d.cs_hard = d.normalisation * coulomb_restricted_cs(mu0, d);
+                    coulomb_normalisation(p.element_Z[iel], p.element_A[iel],
+                        p.mass, kinetic, kinetic0);
+                return d.normalisation * coulomb_restricted_cs(mu0, d);
         }
 
         /*
-         * transport_do_ehs, src/pumas.c:5768-5885. The per element data of
-         * the reference's workspace are recomputed for the selected element
-         * instead of being kept for all elements; the recomputation repeats
-         * the same operations on the same inputs.
+         * transport_do_ehs, src/pumas.c:5768-5885. Cross sections are only
+         * needed to choose among several elements: a single element is
+         * chosen whatever they are (the reference's draw is still made).
+         * The scattering parameters of the chosen element are computed
+         * again instead of being kept for all elements; the recomputation
+         * repeats the same operations on the same inputs.
          */
         PG_NOINLINE void transport_do_ehs(Track & t, int material) const
         {
@@ -1294,28 +1416,31 @@ struct Engine {
                     &invlb1);
 
                 const int nc = p.elements_in[material];
-                double cs_hard[MAX_COMPONENTS];
-                double cs_tot = 0.;
-                for (int i = 0; i < nc; i++) {
-                        CoulombData d;
-                        coulomb_setup(d, material, i, kinetic, mu0);
-                        cs_hard[i] = d.cs_hard;
-                        cs_tot += d.cs_hard;
-                }
-
-                const double cc0 = t.rng.uniform() * cs_tot;
                 int ihard = nc - 1;
-                double cs = 0.;
-                for (int i = 0; i < nc; i++) {
-                        cs += cs_hard[i];
-                        if (cs >= cc0) {
-                                ihard = i;
-                                break;
+                if (nc == 1) {
+                        (void)t.rng.uniform();
+                } else {
+                        double cs_hard[MAX_COMPONENTS];
+                        double cs_tot = 0.;
+                        for (int i = 0; i < nc; i++) {
+                                cs_hard[i] =
+                                    coulomb_cs_hard(material, i, kinetic, mu0);
+                                cs_tot += cs_hard[i];
+                        }
+                        const double cc0 = t.rng.uniform() * cs_tot;
+                        double cs = 0.;
+                        for (int i = 0; i < nc; i++) {
+                                cs += cs_hard[i];
+                                if (cs >= cc0) {
+                                        ihard = i;
+                                        break;
+                                }
                         }
                 }
 
                 CoulombData d;
-                coulomb_setup(d, material, ihard, kinetic, mu0);
+                double kinetic0;
+                coulomb_parameters(d, material, ihard, kinetic, &kinetic0);
                 const int n = d.n_parameters - 1;
                 double A = d.screening[0];
                 for (int i = 1; i < n; i++) {
@@ -1653,6 +1778,9 @@ struct Engine {
                 }
         }
 };
+
+using Engine = EngineT<false>;
+using EngineF32 = EngineT<true>;
 
 } /* namespace pg */
 

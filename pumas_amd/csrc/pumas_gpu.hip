@@ -69,6 +69,7 @@ struct pumas_gpu_physics {
         size_t size;
         PhysicsView view;
         ModelView models;
+        FloatTables floats;     /* FP32 mirrors, in the same allocation */
         int particle;
 };
 
@@ -153,16 +154,42 @@ extern "C" enum pumas_return pumas_gpu_physics_upload_memory(
                 }
         }
 
+        /* Single precision mirrors of the tables read every step by the
+         * FP32-bulk engine, appended after the derived sections. */
+        static const char * const mirrored[] = { "K", "X", "X_dK", "dE",
+                "dE_dK", "Omega", "Omega_dK", "CS", "CS_dK", "Mu0", "Mu0_dK",
+                "Ms1", "Ms1_dK", "Lb", "a_max", "b_max" };
+        const int n_mirrored = sizeof(mirrored) / sizeof(mirrored[0]);
+        std::vector<float> floats;
+        std::vector<size_t> float_at(n_mirrored);
+        for (int k = 0; k < n_mirrored; k++) {
+                const Section * sec = find(sections, mirrored[k]);
+                if ((sec == nullptr) || (sec->dtype != PTAB_F64))
+                        return fail(PUMAS_RETURN_FORMAT_ERROR,
+                            "PTAB section `%s' is missing", mirrored[k]);
+                float_at[k] = floats.size();
+                const double * src =
+                    reinterpret_cast<const double *>(image + sec->offset);
+                for (uint64_t i = 0; i < sec->count; i++)
+                        floats.push_back((float)src[i]);
+                while (floats.size() % 64) floats.push_back(0.f);
+        }
+
         /* Device image: the PTAB file as is, then the derived sections. */
         const size_t extra_offset = (size + 255) / 256 * 256;
-        const size_t total = extra_offset + component0.size() * sizeof(int32_t);
+        const size_t float_offset = (extra_offset +
+            component0.size() * sizeof(int32_t) + 255) / 256 * 256;
+        const size_t total = float_offset + floats.size() * sizeof(float);
         void * device = nullptr;
         HIP_TRY(hipMalloc(&device, total));
         hipError_t e1 = hipMemcpy(device, image, size, hipMemcpyHostToDevice);
         hipError_t e2 = hipMemcpy((uint8_t *)device + extra_offset,
             component0.data(), component0.size() * sizeof(int32_t),
             hipMemcpyHostToDevice);
-        if ((e1 != hipSuccess) || (e2 != hipSuccess)) {
+        hipError_t e3 = hipMemcpy((uint8_t *)device + float_offset,
+            floats.data(), floats.size() * sizeof(float),
+            hipMemcpyHostToDevice);
+        if ((e1 != hipSuccess) || (e2 != hipSuccess) || (e3 != hipSuccess)) {
                 (void)hipFree(device);
                 return fail(PUMAS_RETURN_INTERNAL_ERROR,
                     "could not copy the tables to the device");
@@ -247,6 +274,22 @@ extern "C" enum pumas_return pumas_gpu_physics_upload_memory(
         I32(component_element, "component_element");
         I32(element_index, "element_index");
         v.material_component0 = (const int *)(base + extra_offset);
+
+        FloatTables & ft = ph->floats;
+        {
+                const float * fbase =
+                    (const float *)((const uint8_t *)device + float_offset);
+                const float * const * fields[] = { &ft.K, &ft.X, &ft.X_dK,
+                        &ft.dE, &ft.dE_dK, &ft.Omega, &ft.Omega_dK, &ft.CS,
+                        &ft.CS_dK, &ft.Mu0, &ft.Mu0_dK, &ft.Ms1, &ft.Ms1_dK,
+                        &ft.Lb, &ft.a_max, &ft.b_max };
+                for (int k = 0; k < n_mirrored; k++)
+                        *const_cast<const float **>(fields[k]) =
+                            fbase + float_at[k];
+                ft.n_energies = v.n_energies;
+                ft.n_materials = v.n_materials;
+                ft.mass = (float)v.mass;
+        }
 
         ModelView & mv = ph->models;
         mv.z13 = (const double *)dptr("element_z13");
@@ -475,6 +518,7 @@ extern "C" enum pumas_return pumas_gpu_geometry_create(
 struct KernelArgs {
         PhysicsView p;
         ModelView mv;
+        FloatTables ft;
         Config c;
         pg_geometry g;
         struct pumas_state * states;
@@ -491,7 +535,8 @@ struct KernelArgs {
 };
 
 /* Load history i into a walker. */
-__device__ static void walker_load(const KernelArgs & a, uint64_t i, Walker & w)
+__device__ static void walker_load(const KernelArgs & a, uint64_t i, Walker & w,
+    bool f32)
 {
         const struct pumas_state * s = a.states + i;
         Track & t = w.t;
@@ -514,7 +559,9 @@ __device__ static void walker_load(const KernelArgs & a, uint64_t i, Walker & w)
         t.randn_done = 0;
         t.randn_next = 0.;
         t.ix.reset(a.p.n_energies);
+        t.fx.reset(a.p.n_energies);
         t.rng.init(a.seed, a.first + i, STREAM_TRANSPORT, 0);
+        (void)f32;
         t.steps = 0;
         t.max_steps = a.max_steps;
 }
@@ -567,9 +614,10 @@ __device__ static void walker_store(const KernelArgs & a, uint64_t i,
  * lanes active. Each history still sees its own operations in the
  * reference's order; only the interleaving across lanes changes.
  */
+template <bool F32>
 __global__ void __launch_bounds__(256, 2) transport_persistent(KernelArgs a)
 {
-        const Engine eng = { a.p, a.mv, a.c, a.g };
+        const EngineT<F32> eng = { a.p, a.mv, a.c, a.g, a.ft };
         const int lane = threadIdx.x & 63;
         const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
         Walker w;
@@ -599,7 +647,7 @@ __global__ void __launch_bounds__(256, 2) transport_persistent(KernelArgs a)
                                         index = base +
                                             __popcll(free_lanes & below);
                                         if (index < a.n) {
-                                                walker_load(a, index, w);
+                                                walker_load(a, index, w, F32);
                                                 eng.walker_init(w);
                                         }
                                 }
@@ -675,15 +723,19 @@ extern "C" enum pumas_return pumas_gpu_transport(
                 return fail(PUMAS_RETURN_VALUE_ERROR, "no states (null)");
         enum pumas_return rc = check_context(physics, context);
         if (rc != PUMAS_RETURN_SUCCESS) return rc;
-        if (options && options->precision != PUMAS_GPU_PRECISION_FP64)
+        const int precision =
+            options ? options->precision : PUMAS_GPU_PRECISION_FP64;
+        if ((precision != PUMAS_GPU_PRECISION_FP64) &&
+            (precision != PUMAS_GPU_PRECISION_FP32_BULK))
                 return fail(PUMAS_RETURN_VALUE_ERROR,
-                    "precision mode %d is not available", options->precision);
+                    "unknown precision mode %d", precision);
         hipStream_t stream = static_cast<hipStream_t>(stream_);
 
         KernelArgs a;
         memset(&a, 0x0, sizeof(a));
         a.p = physics->view;
         a.mv = physics->models;
+        a.ft = physics->floats;
         a.c.energy_loss = context->mode.energy_loss;
         a.c.decay = context->mode.decay;
         a.c.direction = context->mode.direction;
@@ -743,8 +795,12 @@ extern "C" enum pumas_return pumas_gpu_transport(
                 HIP_TRY(hipEventRecord(e0, stream));
         }
         if (n > 0) {
-                hipLaunchKernelGGL(transport_persistent, dim3((unsigned)grid),
-                    dim3(block), 0, stream, a);
+                if (precision == PUMAS_GPU_PRECISION_FP32_BULK)
+                        hipLaunchKernelGGL(transport_persistent<true>,
+                            dim3((unsigned)grid), dim3(block), 0, stream, a);
+                else
+                        hipLaunchKernelGGL(transport_persistent<false>,
+                            dim3((unsigned)grid), dim3(block), 0, stream, a);
                 HIP_TRY(hipGetLastError());
         }
         unsigned long long host_counters[4] = { 0, 0, 0, 0 };
