@@ -61,10 +61,10 @@ PG_FN float fpchip(float t, float p0, float p1, float m0, float m1)
         return p0 + t * (m0 + t * (c2 + t * c3));
 }
 
-PG_FN float finterpolate(const FloatTables & f, FloatCache & ch,
+PG_FN float finterpolate(const FloatTables & f, int * last,
     const float * tx, const float * ty, const float * tm, float x)
 {
-        const int i1 = findex(tx, f.n_energies, x, (tx == f.K) ? ch.k : ch.x);
+        const int i1 = findex(tx, f.n_energies, x, last);
         const int i2 = i1 + 1;
         const float dX = tx[i2] - tx[i1];
         const float t = (x - tx[i1]) / dX;
@@ -99,7 +99,7 @@ PG_FN float fcel_grammage(const FloatTables & f, FloatCache & ch, int scheme,
                 return X[imax] +
                     1.f / b * logf((kinetic + K1) / (f.K[imax] + K1));
         }
-        return finterpolate(f, ch, f.K, X, fcol2(f, f.X_dK, scheme, material),
+        return finterpolate(f, ch.k, f.K, X, fcol2(f, f.X_dK, scheme, material),
             kinetic);
 }
 
@@ -113,7 +113,7 @@ PG_FN float fcel_energy_loss(const FloatTables & f, FloatCache & ch,
                 return f.a_max[material] +
                     f.b_max[scheme2(scheme) * f.n_materials + material] *
                     (kinetic + f.mass);
-        return finterpolate(f, ch, f.K, fcol2(f, f.dE, scheme, material),
+        return finterpolate(f, ch.k, f.K, fcol2(f, f.dE, scheme, material),
             fcol2(f, f.dE_dK, scheme, material), kinetic);
 }
 
@@ -129,7 +129,7 @@ PG_FN float fcel_straggling(
                 return Om[imax] * r * r;
         }
         return finterpolate(
-            f, ch, f.K, Om, fcol1(f, f.Omega_dK, material), kinetic);
+            f, ch.k, f.K, Om, fcol1(f, f.Omega_dK, material), kinetic);
 }
 
 /* DEL cross section, as del_cross_section. */
@@ -141,7 +141,7 @@ PG_FN float fdel_cross_section(
         const float * CS = fcol1(f, f.CS, material);
         if (kinetic >= f.K[imax]) return CS[imax];
         return finterpolate(
-            f, ch, f.K, CS, fcol1(f, f.CS_dK, material), kinetic);
+            f, ch.k, f.K, CS, fcol1(f, f.CS_dK, material), kinetic);
 }
 
 /* Multiple scattering parameters, as table_get_msc. */

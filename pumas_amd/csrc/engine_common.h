@@ -323,11 +323,10 @@ PG_FN int index_K(const PhysicsView & p, Cache & ch, double kinetic)
 }
 
 /* src/pumas.c:4146-4157. */
-PG_FN double interpolate(const PhysicsView & p, Cache & ch, const double * tx,
+PG_FN double interpolate(const PhysicsView & p, int * last, const double * tx,
     const double * ty, const double * tm, double x)
 {
-        const int i1 = cached_index(
-            tx, p.n_energies, x, (tx == p.K) ? ch.k : ch.x);
+        const int i1 = cached_index(tx, p.n_energies, x, last);
         const int i2 = i1 + 1;
         const double dX = tx[i2] - tx[i1];
         const double t = (x - tx[i1]) / dX;
@@ -350,7 +349,7 @@ PG_FN double cel_grammage(
                 return *tab2(p, p.X, scheme, material, imax) +
                     1. / b * log((kinetic + K1) / (K0 + K1));
         }
-        return interpolate(p, ch, p.K, tab2(p, p.X, scheme, material, 0),
+        return interpolate(p, ch.k, p.K, tab2(p, p.X, scheme, material, 0),
             tab2(p, p.X_dK, scheme, material, 0), kinetic);
 }
 
@@ -373,7 +372,7 @@ PG_FN double cel_grammage_as_time(
                 return *tab2(p, p.X, scheme, material, imax) +
                     1. / b * log((kinetic + K1) / (E0 - p.mass + K1));
         }
-        return interpolate(p, ch, T, tab2(p, p.X, scheme, material, 0),
+        return interpolate(p, ch.x, T, tab2(p, p.X, scheme, material, 0),
             tab2(p, p.X_dT, scheme, material, 0), time);
 }
 
@@ -391,7 +390,7 @@ PG_FN double cel_proper_time(
                 return *tab2(p, p.T, scheme, material, imax) +
                     p.mass / a * log((E1 / E0) * (a + b * E0) / (a + b * E1));
         }
-        return interpolate(p, ch, p.K, tab2(p, p.T, scheme, material, 0),
+        return interpolate(p, ch.k, p.K, tab2(p, p.T, scheme, material, 0),
             tab2(p, p.T_dK, scheme, material, 0), kinetic);
 }
 
@@ -410,7 +409,7 @@ PG_FN double cel_kinetic_energy(
                 return (K0 + K1) * exp(b * (grammage - X[imax])) - K1;
         }
         return interpolate(
-            p, ch, X, p.K, tab2(p, p.K_dX, scheme, material, 0), grammage);
+            p, ch.x, X, p.K, tab2(p, p.K_dX, scheme, material, 0), grammage);
 }
 
 /* cel_energy_loss, src/pumas.c:3796-3814. */
@@ -423,7 +422,7 @@ PG_FN double cel_energy_loss(
                 return p.a_max[material] +
                     b_max(p, scheme, material) * (kinetic + p.mass);
         }
-        return interpolate(p, ch, p.K, tab2(p, p.dE, scheme, material, 0),
+        return interpolate(p, ch.k, p.K, tab2(p, p.dE, scheme, material, 0),
             tab2(p, p.dE_dK, scheme, material, 0), kinetic);
 }
 
@@ -437,7 +436,7 @@ PG_FN double cel_straggling(const PhysicsView & p, Cache & ch, int material, dou
                 const double r = (kinetic + p.mass) / (Kmax + p.mass);
                 return *tab1(p, p.Omega, material, imax) * r * r;
         }
-        return interpolate(p, ch, p.K, tab1(p, p.Omega, material, 0),
+        return interpolate(p, ch.k, p.K, tab1(p, p.Omega, material, 0),
             tab1(p, p.Omega_dK, material, 0), kinetic);
 }
 
@@ -448,7 +447,7 @@ PG_FN double del_cross_section(
         const int imax = p.n_energies - 1;
         if (kinetic < p.K[0]) return 0.;
         if (kinetic >= p.K[imax]) return *tab1(p, p.CS, material, imax);
-        return interpolate(p, ch, p.K, tab1(p, p.CS, material, 0),
+        return interpolate(p, ch.k, p.K, tab1(p, p.CS, material, 0),
             tab1(p, p.CS_dK, material, 0), kinetic);
 }
 
@@ -468,7 +467,7 @@ PG_FN double del_interaction_length(
                         (a0 + b0 * (k0 + p.mass)));
                 return *tab1(p, p.NI_in, material, imax) + dZ;
         }
-        return interpolate(p, ch, p.K, tab1(p, p.NI_in, material, 0),
+        return interpolate(p, ch.k, p.K, tab1(p, p.NI_in, material, 0),
             tab1(p, p.NI_in_dK, material, 0), kinetic);
 }
 
@@ -489,7 +488,7 @@ PG_FN double del_kinetic_from_interaction_length(
                            a0) / b0 - p.mass;
         }
         return interpolate(
-            p, ch, NI, p.K, tab1(p, p.K_dNI_in, material, 0), nI);
+            p, ch.x, NI, p.K, tab1(p, p.K_dNI_in, material, 0), nI);
 }
 
 /* ehs_interaction_length, src/pumas.c:3998-4023. */
@@ -506,7 +505,7 @@ PG_FN double ehs_interaction_length(
                 const double nI1 = NI[imax];
                 return nI1 + (kinetic - k1) * (nI1 - nI0) / (k1 - k0);
         }
-        return interpolate(p, ch, p.K, NI,
+        return interpolate(p, ch.k, p.K, NI,
             tab_NI_el(p, p.NI_el_dK, scheme, material, 0), kinetic);
 }
 
@@ -526,7 +525,7 @@ PG_FN double ehs_kinetic_from_interaction_length(
         }
         /* K_dNI_el shares the two-scheme layout of the other CEL tables. */
         return interpolate(
-            p, ch, NI, p.K, tab2(p, p.K_dNI_el, scheme, material, 0), nI);
+            p, ch.x, NI, p.K, tab2(p, p.K_dNI_el, scheme, material, 0), nI);
 }
 
 /* table_get_msc, src/pumas.c:4077-4111. */

@@ -556,7 +556,7 @@ template <bool F32> struct EngineT {
 
         /* step_transport, src/pumas.c:6296-6925 (uniform media, no magnetic
          * field). Returns the end medium through `out_medium`. */
-        PG_NOINLINE void step_transport(Track & t, int straight, int medium,
+        PG_FN void step_transport(Track & t, int straight, int medium,
             int material, double density, double grammage_max,
             double step_max_medium, int step_max_type, int * out_medium) const
         {
@@ -1494,8 +1494,15 @@ template <bool F32> struct EngineT {
          * DEL and EHS (a pending discrete event), DONE. The operations and
          * random draws of a history are the reference's, in its order; a
          * scheduler may interleave the phases of many histories freely.
+         *
+         * The phase functions are inlined into the scheduler so that a
+         * Walker stays in registers. The rare, heavy parts (closed form
+         * CSDA, DEL and EHS sampling) are out of line and work on copies of
+         * the Track and of the engine: passing the address of the Walker or
+         * of the scheduler's engine to a call would pin them to scratch
+         * memory for every step.
          */
-        PG_NOINLINE void walker_init(Walker & w) const
+        PG_FN void walker_init(Walker & w) const
         {
                 Track & t = w.t;
                 w.event = EVENT_NONE;
@@ -1558,7 +1565,11 @@ template <bool F32> struct EngineT {
                  * is its start medium. */
                 w.end_medium = medium;
                 if (!do_stepping) {
-                        w.event = transport_with_csda(t, material, density);
+                        Track tmp = t;
+                        const EngineT cold = *this;
+                        w.event = cold.transport_with_csda(
+                            tmp, material, density);
+                        t = tmp;
                         return;
                 }
 
@@ -1669,7 +1680,7 @@ template <bool F32> struct EngineT {
 
         /* One iteration of the step loop up to the event dispatch,
          * src/pumas.c:5277-5467. */
-        PG_NOINLINE void walker_step(Walker & w) const
+        PG_FN void walker_step(Walker & w) const
         {
                 Track & t = w.t;
                 const int scheme = c.energy_loss;
@@ -1738,10 +1749,15 @@ template <bool F32> struct EngineT {
         }
 
         /* A pending inelastic DEL, src/pumas.c:5339-5386. */
-        PG_NOINLINE void walker_del(Walker & w) const
+        PG_FN void walker_del(Walker & w) const
         {
                 Track & t = w.t;
-                transport_do_del(t, w.material);
+                {
+                        Track tmp = t;
+                        const EngineT cold = *this;
+                        cold.transport_do_del(tmp, w.material);
+                        t = tmp;
+                }
                 if ((c.event & t.step_event) ||
                     t.step_event == EVENT_WEIGHT) {
                         walker_finish(w);
@@ -1753,10 +1769,15 @@ template <bool F32> struct EngineT {
         }
 
         /* A pending elastic hard scattering, src/pumas.c:5393-5398. */
-        PG_NOINLINE void walker_ehs(Walker & w) const
+        PG_FN void walker_ehs(Walker & w) const
         {
                 Track & t = w.t;
-                transport_do_ehs(t, w.material);
+                {
+                        Track tmp = t;
+                        const EngineT cold = *this;
+                        cold.transport_do_ehs(tmp, w.material);
+                        t = tmp;
+                }
                 if (c.event & t.step_event) {
                         walker_finish(w);
                         return;

@@ -513,8 +513,37 @@ extern "C" enum pumas_return pumas_gpu_geometry_create(
 }
 
 /* ------------------------------------------------------------------------
- * Transport kernel.
+ * Transport kernels: a wavefront scheduler over per-phase queues.
+ *
+ * A batch runs in rounds. k_init loads every history and runs the entry
+ * logic of pumas_context_transport; histories that need stepping get a
+ * record (their Walker, in HBM) and a slot in the step queue. Each round
+ * then runs three kernels:
+ *   k_step  persistent; a lane takes a slot from the step queue and steps
+ *           that history in registers until it ends or reaches a discrete
+ *           event, then writes it back and appends the slot to the DEL or
+ *           EHS queue (one atomic per wavefront and queue);
+ *   k_ehs   one lane per queued elastic hard scattering;
+ *   k_del   one lane per queued inelastic DEL;
+ * both of which return their histories to the next round's step queue.
+ * The step kernel holds only the stepping code, so its Walker stays in
+ * registers without spills; the rarer, register- and scratch-hungry event
+ * samplers run in their own kernels with every lane busy on the same code.
+ * Each history still sees its own operations in the reference's order: only
+ * the interleaving across histories differs, and the random stream of a
+ * history is keyed by its id, so results do not depend on the schedule.
  * ------------------------------------------------------------------------ */
+enum {
+        C_TAKEN = 0, /* step queue entries taken in this round */
+        C_STEPS,     /* Monte Carlo steps of finished histories */
+        C_ERROR,     /* first error code raised by a history */
+        C_STEP_N0,   /* step queue lengths (ping-pong) */
+        C_STEP_N1,
+        C_EHS_N,     /* EHS queue length */
+        C_DEL_N,     /* DEL queue length */
+        C_COUNT = 8
+};
+
 struct KernelArgs {
         PhysicsView p;
         ModelView mv;
@@ -531,12 +560,24 @@ struct KernelArgs {
         uint32_t * draws;
         int32_t * rc;
         int max_steps;
-        unsigned long long * counters; /* [0] queue, [1] steps, [2] error */
+        unsigned long long * counters;
+        Walker * records;     /* [n] histories in flight */
+        uint32_t * q_step[2]; /* [n] step queues (ping-pong) */
+        uint32_t * q_ehs;     /* [n] */
+        uint32_t * q_del;     /* [n] */
 };
 
+/* The arguments are read in place from the kernarg segment (scalar loads of
+ * uniform data): naming the by-value parameter's members by reference would
+ * make a per-lane scratch copy of it. */
+#define KERNEL_ARGS(args)                                                      \
+        (void)(args);                                                          \
+        const KernelArgs & a =                                                 \
+            *(const KernelArgs *)__builtin_amdgcn_kernarg_segment_ptr()
+
 /* Load history i into a walker. */
-__device__ static void walker_load(const KernelArgs & a, uint64_t i, Walker & w,
-    bool f32)
+__device__ __forceinline__ void walker_load(const KernelArgs & a, uint64_t i,
+    Walker & w)
 {
         const struct pumas_state * s = a.states + i;
         Track & t = w.t;
@@ -561,13 +602,12 @@ __device__ static void walker_load(const KernelArgs & a, uint64_t i, Walker & w,
         t.ix.reset(a.p.n_energies);
         t.fx.reset(a.p.n_energies);
         t.rng.init(a.seed, a.first + i, STREAM_TRANSPORT, 0);
-        (void)f32;
         t.steps = 0;
         t.max_steps = a.max_steps;
 }
 
 /* Store the end state and outputs of history i. */
-__device__ static void walker_store(const KernelArgs & a, uint64_t i,
+__device__ __forceinline__ void walker_store(const KernelArgs & a, uint64_t i,
     const Walker & w)
 {
         struct pumas_state * s = a.states + i;
@@ -594,43 +634,100 @@ __device__ static void walker_store(const KernelArgs & a, uint64_t i,
         if (a.draws) a.draws[i] = t.rng.count;
         if (a.rc) a.rc[i] = w.rc;
         if (w.rc != RETURN_SUCCESS)
-                atomicCAS(a.counters + 2, 0ull, (unsigned long long)w.rc);
+                atomicCAS(a.counters + C_ERROR, 0ull,
+                    (unsigned long long)w.rc);
 }
 
-/* Minimum number of lanes of a wavefront with a pending elastic hard
- * scattering before the wavefront processes them together. */
-#ifndef EHS_BATCH
-#define EHS_BATCH 16
-#endif
-
-/*
- * Persistent transport kernel. A wavefront keeps 64 histories in flight.
- * Every iteration, lanes whose history ended take the next history ids from
- * the global queue (one atomic per wavefront: ballot of the free lanes,
- * prefix count for the ids), then the wavefront advances every history by
- * one phase. Elastic hard scatterings, on ~16% of the steps of a muon in
- * rock, are deferred until EHS_BATCH lanes are waiting for one (or no lane
- * can step), so that the wavefront runs that code path with most of its
- * lanes active. Each history still sees its own operations in the
- * reference's order; only the interleaving across lanes changes.
- */
-template <bool F32>
-__global__ void __launch_bounds__(256, 2) transport_persistent(KernelArgs a)
+__device__ __forceinline__ uint64_t lanes_below()
 {
-        const EngineT<F32> eng = { a.p, a.mv, a.c, a.g, a.ft };
         const int lane = threadIdx.x & 63;
-        const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        return (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+}
+
+/* Append `slot` to `queue` for the lanes where `pred` holds: one atomic per
+ * wavefront, positions by prefix count. Called by all lanes of the wave. */
+__device__ __forceinline__ void queue_append(unsigned long long * count,
+    uint32_t * queue, bool pred, uint32_t slot)
+{
+        const uint64_t m = __ballot(pred);
+        if (!m) return;
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        unsigned long long base = 0;
+        if ((int)(threadIdx.x & 63) == leader)
+                base = atomicAdd(count, (unsigned long long)__popcll(m));
+        base = __shfl(base, leader, 64);
+        if (pred) queue[base + __popcll(m & lanes_below())] = slot;
+}
+
+/* Sum a per-lane step tally over the wave into the global counter. */
+__device__ __forceinline__ void steps_flush(
+    const KernelArgs & a, unsigned long long acc)
+{
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+        if ((threadIdx.x & 63) == 0 && acc) atomicAdd(a.counters + C_STEPS, acc);
+}
+
+/* Entry of every history of the batch; one lane per history. */
+template <bool F32>
+__global__ void __launch_bounds__(256) k_init(KernelArgs args)
+{
+        KERNEL_ARGS(args);
+        const EngineT<F32> eng = { a.p, a.mv, a.c, a.g, a.ft };
+        const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const bool active = i < a.n;
         Walker w;
         w.phase = PHASE_EMPTY;
-        uint64_t index = 0;
+        if (active) {
+                walker_load(a, i, w);
+                eng.walker_init(w);
+                if (w.phase == PHASE_DONE) walker_store(a, i, w);
+                if (w.phase == PHASE_STEP) a.records[i] = w;
+        }
+        queue_append(a.counters + C_STEP_N0, a.q_step[0],
+            w.phase == PHASE_STEP, (uint32_t)i);
+}
+
+/* Clears the per-round counters before round `parity`. */
+__global__ void k_round(unsigned long long * c, int parity)
+{
+        if (threadIdx.x == 0) {
+                c[C_TAKEN] = 0;
+                c[C_EHS_N] = 0;
+                c[C_DEL_N] = 0;
+                c[C_STEP_N0 + (parity ^ 1)] = 0;
+        }
+}
+
+/* Stepping: persistent over the step queue of round `parity`. */
+template <bool F32>
+__global__ void __launch_bounds__(256, 2) k_step(KernelArgs args, int parity)
+{
+        KERNEL_ARGS(args);
+        const EngineT<F32> eng = { a.p, a.mv, a.c, a.g, a.ft };
+        const uint64_t below = lanes_below();
+        const uint32_t * in = a.q_step[parity];
+        const unsigned long long n_in = a.counters[C_STEP_N0 + parity];
+        if (n_in == 0) return;
+        Walker w;
+        w.phase = PHASE_EMPTY;
+        uint32_t slot = 0;
         bool exhausted = false;
         unsigned long long steps_acc = 0;
         for (;;) {
-                if (w.phase == PHASE_DONE) {
-                        walker_store(a, index, w);
+                /* Hand over the histories that left the stepping phase. */
+                const bool done = w.phase == PHASE_DONE;
+                const bool to_ehs = w.phase == PHASE_EHS;
+                const bool to_del = w.phase == PHASE_DEL;
+                if (done) {
+                        walker_store(a, slot, w);
                         steps_acc += w.t.steps;
-                        w.phase = PHASE_EMPTY;
                 }
+                if (to_ehs || to_del) a.records[slot] = w;
+                queue_append(a.counters + C_EHS_N, a.q_ehs, to_ehs, slot);
+                queue_append(a.counters + C_DEL_N, a.q_del, to_del, slot);
+                if (done || to_ehs || to_del) w.phase = PHASE_EMPTY;
+
+                /* Refill the free lanes from the step queue. */
                 if (!exhausted) {
                         const uint64_t free_lanes =
                             __ballot(w.phase == PHASE_EMPTY);
@@ -639,41 +736,74 @@ __global__ void __launch_bounds__(256, 2) transport_persistent(KernelArgs a)
                                 const int leader = __ffsll(
                                     (unsigned long long)free_lanes) - 1;
                                 unsigned long long base = 0;
-                                if (lane == leader)
-                                        base = atomicAdd(a.counters, count);
+                                if ((int)(threadIdx.x & 63) == leader)
+                                        base = atomicAdd(
+                                            a.counters + C_TAKEN, count);
                                 base = __shfl(base, leader, 64);
-                                if (base + count >= a.n) exhausted = true;
+                                if (base + count >= n_in) exhausted = true;
                                 if (w.phase == PHASE_EMPTY) {
-                                        index = base +
+                                        const unsigned long long k = base +
                                             __popcll(free_lanes & below);
-                                        if (index < a.n) {
-                                                walker_load(a, index, w, F32);
-                                                eng.walker_init(w);
+                                        if (k < n_in) {
+                                                slot = in[k];
+                                                w = a.records[slot];
                                         }
                                 }
                         }
                 }
-                if (!__ballot(w.phase != PHASE_EMPTY)) {
+                if (!__ballot(w.phase == PHASE_STEP)) {
                         if (exhausted) break;
                         continue;
                 }
-
                 if (w.phase == PHASE_STEP) eng.walker_step(w);
-                if (w.phase == PHASE_DEL) eng.walker_del(w);
-
-                const uint64_t ehs = __ballot(w.phase == PHASE_EHS);
-                const uint64_t stepping = __ballot(w.phase == PHASE_STEP);
-                if (ehs && ((__popcll(ehs) >= EHS_BATCH) || !stepping)) {
-                        if (w.phase == PHASE_EHS) eng.walker_ehs(w);
-                }
         }
-        for (int off = 32; off > 0; off >>= 1)
-                steps_acc += __shfl_down(steps_acc, off, 64);
-        if (lane == 0) atomicAdd(a.counters + 1, steps_acc);
+        steps_flush(a, steps_acc);
 }
 
-/* Clears the queue and tally counters (a kernel, not a memset node, so
- * that the launch sequence can be captured in a graph). */
+/* Discrete events of round `parity`, one lane per queued history; the
+ * histories go back to the next round's step queue. */
+template <bool F32, int PHASE>
+__global__ void __launch_bounds__(256, 2) k_event(KernelArgs args, int parity)
+{
+        KERNEL_ARGS(args);
+        const EngineT<F32> eng = { a.p, a.mv, a.c, a.g, a.ft };
+        const unsigned long long n =
+            a.counters[(PHASE == PHASE_EHS) ? C_EHS_N : C_DEL_N];
+        const uint32_t * queue = (PHASE == PHASE_EHS) ? a.q_ehs : a.q_del;
+        const int lane = threadIdx.x & 63;
+        const unsigned long long stride = (unsigned long long)gridDim.x *
+            blockDim.x;
+        unsigned long long steps_acc = 0;
+        for (unsigned long long base = (unsigned long long)blockIdx.x *
+                     blockDim.x + (threadIdx.x & ~63u);
+             base < n; base += stride) {
+                const unsigned long long k = base + lane;
+                const bool active = k < n;
+                Walker w;
+                w.phase = PHASE_EMPTY;
+                uint32_t slot = 0;
+                if (active) {
+                        slot = queue[k];
+                        w = a.records[slot];
+                        if (PHASE == PHASE_EHS)
+                                eng.walker_ehs(w);
+                        else
+                                eng.walker_del(w);
+                        if (w.phase == PHASE_DONE) {
+                                walker_store(a, slot, w);
+                                steps_acc += w.t.steps;
+                        } else {
+                                a.records[slot] = w;
+                        }
+                }
+                queue_append(a.counters + C_STEP_N0 + (parity ^ 1),
+                    a.q_step[parity ^ 1], w.phase == PHASE_STEP, slot);
+        }
+        steps_flush(a, steps_acc);
+}
+
+/* Clears the counters (a kernel, not a memset node, so that the launch
+ * sequence can be captured in a graph). */
 __global__ void zero_counters(unsigned long long * c, int n)
 {
         if ((int)threadIdx.x < n) c[threadIdx.x] = 0ull;
@@ -683,6 +813,98 @@ __global__ void zero_counters(unsigned long long * c, int n)
  * C ABI transport.
  * ------------------------------------------------------------------------ */
 namespace {
+
+/* Histories per chunk: the records of a chunk take MAX_CHUNK * sizeof(Walker)
+ * bytes of HBM (about 11 GB). */
+constexpr size_t MAX_CHUNK = size_t(1) << 25;
+
+/* Rounds launched between two reads of the queue lengths. */
+constexpr int ROUNDS_PER_CHECK = 16;
+
+template <bool F32>
+enum pumas_return launch_round(const KernelArgs & a, int parity, int n_cu,
+    int block, int per_cu, hipStream_t stream)
+{
+        hipLaunchKernelGGL(k_round, dim3(1), dim3(64), 0, stream, a.counters,
+            parity);
+        hipLaunchKernelGGL(k_step<F32>, dim3((unsigned)(n_cu * per_cu)),
+            dim3(block), 0, stream, a, parity);
+        hipLaunchKernelGGL((k_event<F32, PHASE_EHS>),
+            dim3((unsigned)(n_cu * 4)), dim3(256), 0, stream, a, parity);
+        hipLaunchKernelGGL((k_event<F32, PHASE_DEL>),
+            dim3((unsigned)(n_cu * 2)), dim3(256), 0, stream, a, parity);
+        HIP_TRY(hipGetLastError());
+        return PUMAS_RETURN_SUCCESS;
+}
+
+/* Transport histories [base, base + m) of the batch described by `a0`. */
+enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
+    size_t m, bool f32, int n_cu, int block, int per_cu, hipStream_t stream,
+    unsigned long long * total_steps, unsigned long long * error)
+{
+        KernelArgs a = a0;
+        a.states = a0.states + base;
+        a.n = m;
+        a.first = a0.first + base;
+        if (a0.events) a.events = a0.events + base;
+        if (a0.media) a.media = a0.media + 2 * base;
+        if (a0.steps) a.steps = a0.steps + base;
+        if (a0.draws) a.draws = a0.draws + base;
+        if (a0.rc) a.rc = a0.rc + base;
+
+        void * mem = nullptr;
+        const size_t b_counters = 256;
+        const size_t b_records = ((m * sizeof(Walker)) + 255) & ~size_t(255);
+        const size_t b_queue = ((m * sizeof(uint32_t)) + 255) & ~size_t(255);
+        HIP_TRY(hipMallocAsync(
+            &mem, b_counters + b_records + 4 * b_queue, stream));
+        char * ptr = static_cast<char *>(mem);
+        a.counters = reinterpret_cast<unsigned long long *>(ptr);
+        a.records = reinterpret_cast<Walker *>(ptr + b_counters);
+        char * q = ptr + b_counters + b_records;
+        a.q_step[0] = reinterpret_cast<uint32_t *>(q);
+        a.q_step[1] = reinterpret_cast<uint32_t *>(q + b_queue);
+        a.q_ehs = reinterpret_cast<uint32_t *>(q + 2 * b_queue);
+        a.q_del = reinterpret_cast<uint32_t *>(q + 3 * b_queue);
+
+        enum pumas_return rc = PUMAS_RETURN_SUCCESS;
+        unsigned long long host[C_COUNT];
+        hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, stream,
+            a.counters, C_COUNT);
+        const unsigned blocks = (unsigned)((m + 255) / 256);
+        if (f32)
+                hipLaunchKernelGGL(k_init<true>, dim3(blocks), dim3(256), 0,
+                    stream, a);
+        else
+                hipLaunchKernelGGL(k_init<false>, dim3(blocks), dim3(256), 0,
+                    stream, a);
+        if (hipGetLastError() != hipSuccess) {
+                (void)hipFreeAsync(mem, stream);
+                return fail(PUMAS_RETURN_VALUE_ERROR,
+                    "transport kernel launch failed");
+        }
+        int parity = 0;
+        for (;;) {
+                HIP_TRY(hipMemcpyAsync(host, a.counters, sizeof(host),
+                    hipMemcpyDeviceToHost, stream));
+                HIP_TRY(hipStreamSynchronize(stream));
+                if (host[C_STEP_N0 + parity] == 0) break;
+                for (int r = 0; r < ROUNDS_PER_CHECK; r++) {
+                        rc = f32 ? launch_round<true>(a, parity, n_cu, block,
+                                       per_cu, stream) :
+                                   launch_round<false>(a, parity, n_cu, block,
+                                       per_cu, stream);
+                        if (rc != PUMAS_RETURN_SUCCESS) break;
+                        parity ^= 1;
+                }
+                if (rc != PUMAS_RETURN_SUCCESS) break;
+        }
+        *total_steps += host[C_STEPS];
+        if ((*error == 0) && (host[C_ERROR] != 0)) *error = host[C_ERROR];
+        HIP_TRY(hipFreeAsync(mem, stream));
+        return rc;
+}
+
 
 enum pumas_return check_context(const pumas_gpu_physics * physics,
     const struct pumas_gpu_context * c)
@@ -774,19 +996,7 @@ extern "C" enum pumas_return pumas_gpu_transport(
                     "block size must be a multiple of 64, at most 256");
         const int per_cu = (options && options->blocks_per_cu > 0) ?
             options->blocks_per_cu :
-            4;
-        size_t grid = (size_t)n_cu * per_cu;
-        const size_t needed = (n + block - 1) / block;
-        if (needed < grid) grid = needed;
-        if (grid == 0) grid = 1;
-
-        unsigned long long * counters = nullptr;
-        HIP_TRY(hipMallocAsync((void **)&counters, 4 * sizeof(unsigned long long),
-            stream));
-        hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, stream,
-            counters, 4);
-        HIP_TRY(hipGetLastError());
-        a.counters = counters;
+            2;
 
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (stats) {
@@ -794,33 +1004,29 @@ extern "C" enum pumas_return pumas_gpu_transport(
                 HIP_TRY(hipEventCreate(&e1));
                 HIP_TRY(hipEventRecord(e0, stream));
         }
-        if (n > 0) {
-                if (precision == PUMAS_GPU_PRECISION_FP32_BULK)
-                        hipLaunchKernelGGL(transport_persistent<true>,
-                            dim3((unsigned)grid), dim3(block), 0, stream, a);
-                else
-                        hipLaunchKernelGGL(transport_persistent<false>,
-                            dim3((unsigned)grid), dim3(block), 0, stream, a);
-                HIP_TRY(hipGetLastError());
+        const bool f32 = (precision == PUMAS_GPU_PRECISION_FP32_BULK);
+        unsigned long long total_steps = 0, error = 0;
+        for (size_t base = 0; base < n; base += MAX_CHUNK) {
+                const size_t m = (n - base < MAX_CHUNK) ? n - base : MAX_CHUNK;
+                rc = transport_chunk(a, base, m, f32, n_cu, block, per_cu,
+                    stream, &total_steps, &error);
+                if (rc != PUMAS_RETURN_SUCCESS) break;
         }
-        unsigned long long host_counters[4] = { 0, 0, 0, 0 };
         if (stats) HIP_TRY(hipEventRecord(e1, stream));
-        HIP_TRY(hipMemcpyAsync(host_counters, counters, sizeof(host_counters),
-            hipMemcpyDeviceToHost, stream));
-        HIP_TRY(hipFreeAsync(counters, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         if (stats) {
                 float ms = 0.f;
                 HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
                 (void)hipEventDestroy(e0);
                 (void)hipEventDestroy(e1);
-                stats->steps = host_counters[1];
+                stats->steps = total_steps;
                 stats->histories = n;
                 stats->kernel_ms = ms;
         }
-        if (host_counters[2] != 0)
-                return fail((enum pumas_return)host_counters[2],
-                    "a history raised error code %llu", host_counters[2]);
+        if (rc != PUMAS_RETURN_SUCCESS) return rc;
+        if (error != 0)
+                return fail((enum pumas_return)error,
+                    "a history raised error code %llu", error);
         return PUMAS_RETURN_SUCCESS;
 }
 
