@@ -52,7 +52,24 @@ def parse():
                     help="fp64: reference arithmetic; fp32: FP32-bulk")
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only run a plain device-to-device copy of this "
+                         "many MiB (calibrates HBM byte counters)")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """A plain copy kernel of known size, for scaling HBM byte counters."""
+    import torch
+    n = mib << 20
+    src = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n,
+                      "launches": 3}))
 
 
 def cpu_baseline(states, seconds):
@@ -86,6 +103,9 @@ def cpu_baseline(states, seconds):
 
 def main():
     args = parse()
+    if args.copy_mib > 0:
+        copy_calibration(args.copy_mib)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU "
                           "sharding is described in DESIGN.md and not "
@@ -120,28 +140,34 @@ def main():
             block_size=args.block_size, blocks_per_cu=args.blocks_per_cu)
 
     for k in range(args.warmup):
-        s, ms = one_pass(1000 + k)
-        log(f"bench: warmup pass {k}: {ms:.1f} ms, {s} steps")
+        st = one_pass(1000 + k)
+        log(f"bench: warmup pass {k}: {st.kernel_ms:.1f} ms, {st.steps} steps")
 
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    total_steps, kernel_ms = 0, []
+    passes = []
     for k in range(args.steps):
-        s, ms = one_pass(k)
-        total_steps += s
-        kernel_ms.append(ms)
-        log(f"bench: pass {k}: {ms:.1f} ms, {s} steps")
+        st = one_pass(k)
+        passes.append(st)
+        log(f"bench: pass {k}: {st.kernel_ms:.1f} ms, {st.steps} steps, "
+            f"stepping kernel {st.step_kernel_ms:.1f} ms over "
+            f"{st.step_launches} launches")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
 
+    total_steps = sum(st.steps for st in passes)
     steps_per_s = total_steps / elapsed
     events_per_s = args.steps * n / elapsed
-    mean_ms = sum(kernel_ms) / len(kernel_ms)
-    steps_per_launch = total_steps / args.steps
-    achieved = steps_per_launch * BYTES_PER_STEP_FP64 / (mean_ms * 1e-3) / 1e9
+    mean_ms = sum(st.kernel_ms for st in passes) / len(passes)
+    # Roofline of the dominant kernel (the stepping kernel): algorithmic
+    # bytes of its launches over their summed duration (HIP events around
+    # each launch, on the stream it runs on).
+    step_ms = sum(st.step_kernel_ms for st in passes)
+    launches = sum(st.step_launches for st in passes)
+    achieved = total_steps * BYTES_PER_STEP_FP64 / (step_ms * 1e-3) / 1e9
 
     traffic = None
-    tpath = os.path.join(HERE, "profiles", "transport_persistent_slabs_traffic.json")
+    tpath = os.path.join(HERE, "profiles", "k_step_slabs_traffic.json")
     if os.path.exists(tpath) and n == 10_000_000:
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
@@ -181,10 +207,12 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "transport_persistent",
-            "kernel_ms": mean_ms,
+            "kernel": "k_step",
+            "kernel_avg_ms": step_ms / max(launches, 1),
+            "launches_per_pass": launches / args.steps,
             "bytes_per_step": BYTES_PER_STEP_FP64,
         },
+        "transport_ms_per_pass": mean_ms,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -251,6 +251,9 @@ struct pumas_gpu_options {
         int blocks_per_cu;
         /* Safety cap on Monte Carlo steps per history (0 = 10^7). */
         int max_steps;
+        /* Scheduler rounds launched between two reads of the queue
+         * lengths (0 = default). */
+        int rounds_per_check;
 };
 
 /* Optional per-history outputs (device pointers, each may be NULL). */
@@ -267,6 +270,9 @@ struct pumas_gpu_stats {
         uint64_t steps;         /* total Monte Carlo steps */
         uint64_t histories;     /* histories transported */
         double kernel_ms;       /* transport kernel time (HIP events) */
+        double step_kernel_ms;  /* time in the stepping kernel */
+        uint64_t step_launches; /* launches of the stepping kernel */
+        uint64_t rounds;        /* scheduler rounds with work */
 };
 
 /*

@@ -269,12 +269,31 @@ class TransportResult:
     kernel_ms: float = 0.0
 
 
-def _options(precision, block_size, blocks_per_cu, max_steps):
+@dataclasses.dataclass
+class TransportStats:
+    """Counters of one pumas_gpu_transport call."""
+    steps: int = 0
+    histories: int = 0
+    kernel_ms: float = 0.0
+    step_kernel_ms: float = 0.0
+    step_launches: int = 0
+    rounds: int = 0
+
+    @classmethod
+    def from_c(cls, st):
+        return cls(int(st.steps), int(st.histories), float(st.kernel_ms),
+                   float(st.step_kernel_ms), int(st.step_launches),
+                   int(st.rounds))
+
+
+def _options(precision, block_size, blocks_per_cu, max_steps,
+             rounds_per_check=0):
     o = _abi.GpuOptions()
     o.precision = int(precision)
     o.block_size = block_size
     o.blocks_per_cu = blocks_per_cu
     o.max_steps = max_steps
+    o.rounds_per_check = rounds_per_check
     return o
 
 
@@ -324,19 +343,20 @@ def transport(physics, context, geometry, states, seed=0, first_history=0,
 def transport_device(physics, context, geometry, states_ptr, n, seed=0,
                      first_history=0, outputs=None, precision=Precision.FP64,
                      block_size=0, blocks_per_cu=0, max_steps=0, stream=None,
-                     stats=True):
+                     stats=True, rounds_per_check=0):
     """Transport `n` states already in device memory at `states_ptr`.
 
     `outputs` maps any of "events", "media", "steps", "draws", "rc" to a
     device pointer. `stream` is a hipStream_t handle (int) or None.
-    Returns (total_steps, kernel_ms) when `stats`, else None.
+    Returns a TransportStats when `stats`, else None.
     """
     lib = library()
     out = _abi.GpuOutputs()
     for k, v in (outputs or {}).items():
         setattr(out, k, v)
     st = _abi.GpuStats()
-    opts = _options(precision, block_size, blocks_per_cu, max_steps)
+    opts = _options(precision, block_size, blocks_per_cu, max_steps,
+                    rounds_per_check)
     ctx = context.to_c()
     _check(lib.pumas_gpu_transport(
         physics.handle, ctypes.byref(ctx), geometry.handle,
@@ -345,5 +365,5 @@ def transport_device(physics, context, geometry, states_ptr, n, seed=0,
         ctypes.byref(st) if stats else None,
         ctypes.c_void_p(stream) if stream else None))
     if stats:
-        return int(st.steps), float(st.kernel_ms)
+        return TransportStats.from_c(st)
     return None

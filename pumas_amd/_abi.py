@@ -145,12 +145,16 @@ class GpuOutputs(ctypes.Structure):
 
 class GpuOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("block_size", ctypes.c_int),
-                ("blocks_per_cu", ctypes.c_int), ("max_steps", ctypes.c_int)]
+                ("blocks_per_cu", ctypes.c_int), ("max_steps", ctypes.c_int),
+                ("rounds_per_check", ctypes.c_int)]
 
 
 class GpuStats(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_uint64), ("histories", ctypes.c_uint64),
-                ("kernel_ms", ctypes.c_double)]
+                ("kernel_ms", ctypes.c_double),
+                ("step_kernel_ms", ctypes.c_double),
+                ("step_launches", ctypes.c_uint64),
+                ("rounds", ctypes.c_uint64)]
 
 
 # Every entry point declared by include/pumas_gpu.h.

@@ -30,10 +30,9 @@ struct FloatCache {
         }
 };
 
-PG_FN int findex(const float * table, int n, float value, int * last)
+/* Bracketing row of a value missed by the first cached row. */
+PG_FN int findex_miss(const float * table, int n, float value, int * last)
 {
-        if ((value >= table[last[0]]) && (value < table[last[0] + 1]))
-                return last[0];
         if ((value >= table[last[1]]) && (value < table[last[1] + 1]))
                 return last[1];
         const int imax = n - 1;
@@ -61,16 +60,29 @@ PG_FN float fpchip(float t, float p0, float p1, float m0, float m1)
         return p0 + t * (m0 + t * (c2 + t * c3));
 }
 
+/* Interpolation with speculative rows at the first cached bracket (one
+ * round trip to memory on a cache hit), as interpolate. */
 PG_FN float finterpolate(const FloatTables & f, int * last,
     const float * tx, const float * ty, const float * tm, float x)
 {
-        const int i1 = findex(tx, f.n_energies, x, last);
-        const int i2 = i1 + 1;
-        const float dX = tx[i2] - tx[i1];
-        const float t = (x - tx[i1]) / dX;
-        const float m1 = tm[i1] * dX;
-        const float m2 = (i2 > 1) ? tm[i2] * dX : m1;
-        return fpchip(t, ty[i1], ty[i2], m1, m2);
+        int i1 = last[0];
+        float x0 = tx[i1], x1 = tx[i1 + 1];
+        float y0 = ty[i1], y1 = ty[i1 + 1];
+        float d0 = tm[i1], d1 = tm[i1 + 1];
+        if (!((x >= x0) && (x < x1))) {
+                i1 = findex_miss(tx, f.n_energies, x, last);
+                x0 = tx[i1];
+                x1 = tx[i1 + 1];
+                y0 = ty[i1];
+                y1 = ty[i1 + 1];
+                d0 = tm[i1];
+                d1 = tm[i1 + 1];
+        }
+        const float dX = x1 - x0;
+        const float t = (x - x0) / dX;
+        const float m1 = d0 * dX;
+        const float m2 = (i1 + 1 > 1) ? d1 * dX : m1;
+        return fpchip(t, y0, y1, m1, m2);
 }
 
 PG_FN const float * fcol2(
@@ -160,15 +172,30 @@ PG_FN void ftable_get_msc(const FloatTables & f, FloatCache & ch, int scheme,
                 *mu0 = Mu0[imax];
                 *invlb1 = Ms1[imax] * f.K[imax] / kinetic;
         } else {
-                const int i1 = findex(f.K, f.n_energies, kinetic, ch.k);
-                const float * m0 = fcol1(f, f.Mu0_dK, material) + i1;
+                const float * m0 = fcol1(f, f.Mu0_dK, material);
                 const float * m1 = f.Ms1_dK +
-                    (s * f.n_materials + material) * f.n_energies + i1;
-                const float dx = f.K[i1 + 1] - f.K[i1];
-                const float t = (kinetic - f.K[i1]) / dx;
-                *mu0 = fpchip(t, Mu0[i1], Mu0[i1 + 1], m0[0] * dx, m0[1] * dx);
-                *invlb1 = fpchip(
-                    t, Ms1[i1], Ms1[i1 + 1], m1[0] * dx, m1[1] * dx);
+                    (s * f.n_materials + material) * f.n_energies;
+                int i = ch.k[0];
+                float x0 = f.K[i], x1 = f.K[i + 1];
+                float a0 = Mu0[i], a1 = Mu0[i + 1], b0 = m0[i], b1 = m0[i + 1];
+                float c0 = Ms1[i], c1 = Ms1[i + 1], e0 = m1[i], e1 = m1[i + 1];
+                if (!((kinetic >= x0) && (kinetic < x1))) {
+                        i = findex_miss(f.K, f.n_energies, kinetic, ch.k);
+                        x0 = f.K[i];
+                        x1 = f.K[i + 1];
+                        a0 = Mu0[i];
+                        a1 = Mu0[i + 1];
+                        b0 = m0[i];
+                        b1 = m0[i + 1];
+                        c0 = Ms1[i];
+                        c1 = Ms1[i + 1];
+                        e0 = m1[i];
+                        e1 = m1[i + 1];
+                }
+                const float dx = x1 - x0;
+                const float t = (kinetic - x0) / dx;
+                *mu0 = fpchip(t, a0, a1, b0 * dx, b1 * dx);
+                *invlb1 = fpchip(t, c0, c1, e0 * dx, e1 * dx);
         }
 }
 
@@ -181,9 +208,17 @@ PG_FN float fcoulomb_ehs_length(
         const float * Lb = fcol1(f, f.Lb, material);
         if (kinetic < f.K[1]) return Lb[1] / p2;
         if (kinetic >= f.K[imax]) return Lb[imax] / p2;
-        const int i1 = findex(f.K, f.n_energies, kinetic, ch.k);
-        const float h = (kinetic - f.K[i1]) / (f.K[i1 + 1] - f.K[i1]);
-        return (Lb[i1] + h * (Lb[i1 + 1] - Lb[i1])) / p2;
+        int i = ch.k[0];
+        float x0 = f.K[i], x1 = f.K[i + 1], l0 = Lb[i], l1 = Lb[i + 1];
+        if (!((kinetic >= x0) && (kinetic < x1))) {
+                i = findex_miss(f.K, f.n_energies, kinetic, ch.k);
+                x0 = f.K[i];
+                x1 = f.K[i + 1];
+                l0 = Lb[i];
+                l1 = Lb[i + 1];
+        }
+        const float h = (kinetic - x0) / (x1 - x0);
+        return (l0 + h * (l1 - l0)) / p2;
 }
 
 } /* namespace pg */

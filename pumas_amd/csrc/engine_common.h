@@ -300,10 +300,10 @@ struct Cache {
         }
 };
 
-PG_FN int cached_index(const double * table, int n, double value, int * last)
+/* Bracketing row of a value missed by the first cached row. */
+PG_FN int cached_index_miss(
+    const double * table, int n, double value, int * last)
 {
-        if ((value >= table[last[0]]) && (value < table[last[0] + 1]))
-                return last[0];
         if ((value >= table[last[1]]) && (value < table[last[1] + 1]))
                 return last[1];
         const int i1 = table_index(table, n, value);
@@ -316,6 +316,13 @@ PG_FN int cached_index(const double * table, int n, double value, int * last)
         return i1;
 }
 
+PG_FN int cached_index(const double * table, int n, double value, int * last)
+{
+        if ((value >= table[last[0]]) && (value < table[last[0] + 1]))
+                return last[0];
+        return cached_index_miss(table, n, value, last);
+}
+
 /* Bracketing row in the kinetic energy grid. */
 PG_FN int index_K(const PhysicsView & p, Cache & ch, double kinetic)
 {
@@ -323,16 +330,33 @@ PG_FN int index_K(const PhysicsView & p, Cache & ch, double kinetic)
 }
 
 /* src/pumas.c:4146-4157. */
+/*
+ * The rows of the first cached bracket are loaded together with its bounds,
+ * before the bracket is checked, so that a cache hit (the common case: the
+ * kinetic energy changes little between lookups) costs one round trip to
+ * memory instead of two. A miss reloads them.
+ */
 PG_FN double interpolate(const PhysicsView & p, int * last, const double * tx,
     const double * ty, const double * tm, double x)
 {
-        const int i1 = cached_index(tx, p.n_energies, x, last);
-        const int i2 = i1 + 1;
-        const double dX = tx[i2] - tx[i1];
-        const double t = (x - tx[i1]) / dX;
-        const double m1 = tm[i1] * dX;
-        const double m2 = (i2 > 1) ? tm[i2] * dX : m1;
-        return pchip(t, ty[i1], ty[i2], m1, m2);
+        int i1 = last[0];
+        double x0 = tx[i1], x1 = tx[i1 + 1];
+        double y0 = ty[i1], y1 = ty[i1 + 1];
+        double d0 = tm[i1], d1 = tm[i1 + 1];
+        if (!((x >= x0) && (x < x1))) {
+                i1 = cached_index_miss(tx, p.n_energies, x, last);
+                x0 = tx[i1];
+                x1 = tx[i1 + 1];
+                y0 = ty[i1];
+                y1 = ty[i1 + 1];
+                d0 = tm[i1];
+                d1 = tm[i1 + 1];
+        }
+        const double dX = x1 - x0;
+        const double t = (x - x0) / dX;
+        const double m1 = d0 * dX;
+        const double m2 = (i1 + 1 > 1) ? d1 * dX : m1;
+        return pchip(t, y0, y1, m1, m2);
 }
 
 /* cel_grammage, src/pumas.c:3644-3669. */
@@ -542,17 +566,33 @@ PG_FN void table_get_msc(const PhysicsView & p, Cache & ch, int scheme, int mate
                 *invlb1 = *tab_Ms1(p, p.Ms1, scheme, material, imax) *
                     p.K[imax] / kinetic;
         } else {
-                const double * x = p.K;
-                const int i1 = index_K(p, ch, kinetic);
-                x += i1;
-                const double * y0 = tab1(p, p.Mu0, material, i1);
-                const double * m0 = tab1(p, p.Mu0_dK, material, i1);
-                const double * y1 = tab_Ms1(p, p.Ms1, scheme, material, i1);
-                const double * m1 = tab_Ms1(p, p.Ms1_dK, scheme, material, i1);
-                const double dx = x[1] - x[0];
-                const double t = (kinetic - x[0]) / dx;
-                *mu0 = pchip(t, y0[0], y0[1], m0[0] * dx, m0[1] * dx);
-                *invlb1 = pchip(t, y1[0], y1[1], m1[0] * dx, m1[1] * dx);
+                /* Speculative rows at the first cached bracket, as in
+                 * interpolate. */
+                const double * y0 = tab1(p, p.Mu0, material, 0);
+                const double * m0 = tab1(p, p.Mu0_dK, material, 0);
+                const double * y1 = tab_Ms1(p, p.Ms1, scheme, material, 0);
+                const double * m1 = tab_Ms1(p, p.Ms1_dK, scheme, material, 0);
+                int i = ch.k[0];
+                double x0 = p.K[i], x1 = p.K[i + 1];
+                double a0 = y0[i], a1 = y0[i + 1], b0 = m0[i], b1 = m0[i + 1];
+                double c0 = y1[i], c1 = y1[i + 1], e0 = m1[i], e1 = m1[i + 1];
+                if (!((kinetic >= x0) && (kinetic < x1))) {
+                        i = cached_index_miss(p.K, p.n_energies, kinetic, ch.k);
+                        x0 = p.K[i];
+                        x1 = p.K[i + 1];
+                        a0 = y0[i];
+                        a1 = y0[i + 1];
+                        b0 = m0[i];
+                        b1 = m0[i + 1];
+                        c0 = y1[i];
+                        c1 = y1[i + 1];
+                        e0 = m1[i];
+                        e1 = m1[i + 1];
+                }
+                const double dx = x1 - x0;
+                const double t = (kinetic - x0) / dx;
+                *mu0 = pchip(t, a0, a1, b0 * dx, b1 * dx);
+                *invlb1 = pchip(t, c0, c1, e0 * dx, e1 * dx);
         }
 }
 
@@ -568,10 +608,18 @@ PG_FN double coulomb_ehs_length(
         } else if (kinetic >= p.K[imax]) {
                 return Lb[imax] / p2;
         } else {
-                const int i1 = index_K(p, ch, kinetic);
-                const int i2 = i1 + 1;
-                const double h = (kinetic - p.K[i1]) / (p.K[i2] - p.K[i1]);
-                return (Lb[i1] + h * (Lb[i2] - Lb[i1])) / p2;
+                int i = ch.k[0];
+                double x0 = p.K[i], x1 = p.K[i + 1];
+                double l0 = Lb[i], l1 = Lb[i + 1];
+                if (!((kinetic >= x0) && (kinetic < x1))) {
+                        i = cached_index_miss(p.K, p.n_energies, kinetic, ch.k);
+                        x0 = p.K[i];
+                        x1 = p.K[i + 1];
+                        l0 = Lb[i];
+                        l1 = Lb[i + 1];
+                }
+                const double h = (kinetic - x0) / (x1 - x0);
+                return (l0 + h * (l1 - l0)) / p2;
         }
 }
 

@@ -1408,7 +1408,7 @@ template <bool F32> struct EngineT {
          * again instead of being kept for all elements; the recomputation
          * repeats the same operations on the same inputs.
          */
-        PG_NOINLINE void transport_do_ehs(Track & t, int material) const
+        PG_FN void transport_do_ehs(Track & t, int material) const
         {
                 const double kinetic = t.energy;
                 double mu0 = 0., invlb1 = 0.;
@@ -1495,12 +1495,11 @@ template <bool F32> struct EngineT {
          * random draws of a history are the reference's, in its order; a
          * scheduler may interleave the phases of many histories freely.
          *
-         * The phase functions are inlined into the scheduler so that a
-         * Walker stays in registers. The rare, heavy parts (closed form
-         * CSDA, DEL and EHS sampling) are out of line and work on copies of
+         * The phase functions are inlined into the scheduler kernels so
+         * that a Walker stays in registers. The rare, heavy parts (closed
+         * form CSDA and DEL sampling) are out of line and work on copies of
          * the Track and of the engine: passing the address of the Walker or
-         * of the scheduler's engine to a call would pin them to scratch
-         * memory for every step.
+         * of the kernel's engine to a call would pin them to scratch memory.
          */
         PG_FN void walker_init(Walker & w) const
         {
@@ -1773,10 +1772,8 @@ template <bool F32> struct EngineT {
         {
                 Track & t = w.t;
                 {
-                        Track tmp = t;
-                        const EngineT cold = *this;
-                        cold.transport_do_ehs(tmp, w.material);
-                        t = tmp;
+                        /* Inlined: the EHS kernel holds nothing else. */
+                        transport_do_ehs(t, w.material);
                 }
                 if (c.event & t.step_event) {
                         walker_finish(w);

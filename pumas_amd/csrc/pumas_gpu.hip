@@ -818,17 +818,30 @@ namespace {
  * bytes of HBM (about 11 GB). */
 constexpr size_t MAX_CHUNK = size_t(1) << 25;
 
-/* Rounds launched between two reads of the queue lengths. */
+/* Default number of rounds launched between two reads of the queue
+ * lengths, and its maximum. */
 constexpr int ROUNDS_PER_CHECK = 16;
+constexpr int MAX_ROUNDS_PER_CHECK = 256;
+
+/* Launch geometry and optional timing of the stepping kernel. */
+struct Launch {
+        int n_cu, block, per_cu, rounds_per_check;
+        hipStream_t stream;
+        hipEvent_t * marks; /* [2 * rounds_per_check] or NULL */
+};
 
 template <bool F32>
-enum pumas_return launch_round(const KernelArgs & a, int parity, int n_cu,
-    int block, int per_cu, hipStream_t stream)
+enum pumas_return launch_round(
+    const KernelArgs & a, int parity, const Launch & L, int r)
 {
+        hipStream_t stream = L.stream;
         hipLaunchKernelGGL(k_round, dim3(1), dim3(64), 0, stream, a.counters,
             parity);
-        hipLaunchKernelGGL(k_step<F32>, dim3((unsigned)(n_cu * per_cu)),
-            dim3(block), 0, stream, a, parity);
+        if (L.marks) HIP_TRY(hipEventRecord(L.marks[2 * r], stream));
+        hipLaunchKernelGGL(k_step<F32>, dim3((unsigned)(L.n_cu * L.per_cu)),
+            dim3(L.block), 0, stream, a, parity);
+        if (L.marks) HIP_TRY(hipEventRecord(L.marks[2 * r + 1], stream));
+        const int n_cu = L.n_cu;
         hipLaunchKernelGGL((k_event<F32, PHASE_EHS>),
             dim3((unsigned)(n_cu * 4)), dim3(256), 0, stream, a, parity);
         hipLaunchKernelGGL((k_event<F32, PHASE_DEL>),
@@ -839,9 +852,10 @@ enum pumas_return launch_round(const KernelArgs & a, int parity, int n_cu,
 
 /* Transport histories [base, base + m) of the batch described by `a0`. */
 enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
-    size_t m, bool f32, int n_cu, int block, int per_cu, hipStream_t stream,
-    unsigned long long * total_steps, unsigned long long * error)
+    size_t m, bool f32, const Launch & L, unsigned long long * total_steps,
+    unsigned long long * error, struct pumas_gpu_stats * stats)
 {
+        hipStream_t stream = L.stream;
         KernelArgs a = a0;
         a.states = a0.states + base;
         a.n = m;
@@ -883,18 +897,28 @@ enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
                 return fail(PUMAS_RETURN_VALUE_ERROR,
                     "transport kernel launch failed");
         }
-        int parity = 0;
+        int parity = 0, launched = 0;
         for (;;) {
                 HIP_TRY(hipMemcpyAsync(host, a.counters, sizeof(host),
                     hipMemcpyDeviceToHost, stream));
                 HIP_TRY(hipStreamSynchronize(stream));
+                if (L.marks && stats) {
+                        for (int r = 0; r < launched; r++) {
+                                float ms = 0.f;
+                                HIP_TRY(hipEventElapsedTime(&ms,
+                                    L.marks[2 * r], L.marks[2 * r + 1]));
+                                stats->step_kernel_ms += ms;
+                        }
+                }
+                if (stats) stats->step_launches += launched;
+                launched = 0;
                 if (host[C_STEP_N0 + parity] == 0) break;
-                for (int r = 0; r < ROUNDS_PER_CHECK; r++) {
-                        rc = f32 ? launch_round<true>(a, parity, n_cu, block,
-                                       per_cu, stream) :
-                                   launch_round<false>(a, parity, n_cu, block,
-                                       per_cu, stream);
+                if (stats) stats->rounds += 1;
+                for (int r = 0; r < L.rounds_per_check; r++) {
+                        rc = f32 ? launch_round<true>(a, parity, L, r) :
+                                   launch_round<false>(a, parity, L, r);
                         if (rc != PUMAS_RETURN_SUCCESS) break;
+                        launched++;
                         parity ^= 1;
                 }
                 if (rc != PUMAS_RETURN_SUCCESS) break;
@@ -997,6 +1021,14 @@ extern "C" enum pumas_return pumas_gpu_transport(
         const int per_cu = (options && options->blocks_per_cu > 0) ?
             options->blocks_per_cu :
             2;
+        const int rounds_per_check =
+            (options && options->rounds_per_check > 0) ?
+            options->rounds_per_check :
+            ROUNDS_PER_CHECK;
+        if (rounds_per_check > MAX_ROUNDS_PER_CHECK)
+                return fail(PUMAS_RETURN_VALUE_ERROR,
+                    "rounds_per_check must be at most %d",
+                    MAX_ROUNDS_PER_CHECK);
 
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (stats) {
@@ -1006,14 +1038,23 @@ extern "C" enum pumas_return pumas_gpu_transport(
         }
         const bool f32 = (precision == PUMAS_GPU_PRECISION_FP32_BULK);
         unsigned long long total_steps = 0, error = 0;
+        std::vector<hipEvent_t> marks;
+        if (stats) {
+                memset(stats, 0x0, sizeof(*stats));
+                marks.resize(2 * rounds_per_check, nullptr);
+                for (auto & e : marks) HIP_TRY(hipEventCreate(&e));
+        }
+        const Launch L = { n_cu, block, per_cu, rounds_per_check, stream,
+                stats ? marks.data() : nullptr };
         for (size_t base = 0; base < n; base += MAX_CHUNK) {
                 const size_t m = (n - base < MAX_CHUNK) ? n - base : MAX_CHUNK;
-                rc = transport_chunk(a, base, m, f32, n_cu, block, per_cu,
-                    stream, &total_steps, &error);
+                rc = transport_chunk(
+                    a, base, m, f32, L, &total_steps, &error, stats);
                 if (rc != PUMAS_RETURN_SUCCESS) break;
         }
         if (stats) HIP_TRY(hipEventRecord(e1, stream));
         HIP_TRY(hipStreamSynchronize(stream));
+        for (auto & e : marks) (void)hipEventDestroy(e);
         if (stats) {
                 float ms = 0.f;
                 HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
