@@ -1,0 +1,97 @@
+/*
+ * Test infrastructure: the device transport engine (pumas_amd/csrc/engine_*.h)
+ * compiled for the host CPU with g++, so that its arithmetic can be compared
+ * bit for bit with the oracle on the CPU. The GPU build differs from this one
+ * only in its libm (OCML instead of glibc). This file is not part of the
+ * library and is not reachable from its transport path.
+ */
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../include/pumas_gpu.h"
+#include "engine_transport.h"
+#include "ptab.h"
+
+using namespace pg;
+
+static std::vector<uint8_t> g_image;
+static PhysicsView g_view;
+static ModelView g_mv;
+static std::vector<int> g_c0;
+
+static const void *sec(const char *name) {
+    const ptab_header *h = (const ptab_header *)g_image.data();
+    const ptab_entry *e = (const ptab_entry *)(g_image.data() + sizeof(ptab_header));
+    for (uint32_t i = 0; i < h->n_entries; i++)
+        if (!strncmp(e[i].name, name, PTAB_NAME_LEN)) return g_image.data() + e[i].offset;
+    fprintf(stderr, "missing %s\n", name); return nullptr;
+}
+
+extern "C" int host_load(const char *path) {
+    FILE *f = fopen(path, "rb"); if (!f) return -1;
+    fseek(f, 0, SEEK_END); long n = ftell(f); fseek(f, 0, SEEK_SET);
+    g_image.resize(n); if (fread(g_image.data(), 1, n, f) != (size_t)n) return -1; fclose(f);
+    const ptab_header *h = (const ptab_header *)g_image.data();
+    PhysicsView &v = g_view;
+    v.n_energies = h->imeta[PTAB_N_ENERGIES]; v.n_materials = h->imeta[PTAB_N_MATERIALS];
+    v.n_elements = h->imeta[PTAB_N_ELEMENTS]; v.n_components = h->imeta[PTAB_N_COMPONENTS];
+    v.n_table_dcs = h->imeta[PTAB_N_TABLE_DCS]; v.max_components = h->imeta[PTAB_MAX_COMPONENTS];
+    v.mass = h->fmeta[PTAB_MASS]; v.ctau = h->fmeta[PTAB_CTAU]; v.cutoff = h->fmeta[PTAB_CUTOFF];
+#define D(f, n) v.f = (const double *)sec(n)
+    D(K,"K"); D(K_dX,"K_dX"); D(K_dNI_in,"K_dNI_in"); D(K_dNI_el,"K_dNI_el"); D(X,"X"); D(X_dK,"X_dK");
+    D(X_dT,"X_dT"); D(T,"T"); D(T_dK,"T_dK"); D(dE,"dE"); D(dE_dK,"dE_dK"); D(Omega,"Omega");
+    D(Omega_dK,"Omega_dK"); D(NI_el,"NI_el"); D(NI_el_dK,"NI_el_dK"); D(NI_in,"NI_in"); D(NI_in_dK,"NI_in_dK");
+    D(CS,"CS"); D(CS_dK,"CS_dK"); D(CSf,"CSf"); D(CSf_dK,"CSf_dK"); D(CSn,"CSn"); D(CSn_dK,"CSn_dK");
+    D(Kt,"Kt"); D(a_max,"a_max"); D(b_max,"b_max"); D(Mu0,"Mu0"); D(Mu0_dK,"Mu0_dK"); D(Lb,"Lb");
+    D(Lb_dK,"Lb_dK"); D(Ms1,"Ms1"); D(Ms1_dK,"Ms1_dK"); D(material_density,"material_density");
+    D(component_fraction,"component_fraction"); D(element_Z,"element_Z"); D(element_A,"element_A");
+    D(element_I,"element_I"); D(element_nscr,"element_n_screening"); D(element_amplitude,"element_amplitude");
+    D(element_exponent,"element_exponent"); D(element_rn,"element_nuclear_radius"); D(element_lrad,"element_radiation_log");
+    v.DCS = (const float*)sec("DCS"); v.DCS_x = (const float*)sec("DCS_x"); v.DCS_envelope = (const float*)sec("DCS_envelope");
+    v.elements_in = (const int*)sec("elements_in"); v.component_element = (const int*)sec("component_element");
+    v.element_index = (const int*)sec("element_index");
+    g_c0.resize(v.n_materials); int acc = 0;
+    for (int m = 0; m < v.n_materials; m++) { g_c0[m] = acc; acc += v.elements_in[m]; }
+    v.material_component0 = g_c0.data();
+    g_mv.z13 = (const double*)sec("element_z13"); g_mv.zm13 = (const double*)sec("element_zm13");
+    g_mv.z23 = (const double*)sec("element_z23"); g_mv.dn = (const double*)sec("element_dn");
+    g_mv.lrad = v.element_lrad; g_mv.cst = (const double*)sec("constants");
+    g_mv.gq9_x = (const double*)sec("gq9_x"); g_mv.gq9_w = (const double*)sec("gq9_w");
+    g_mv.gq12_x = (const double*)sec("gq12_x"); g_mv.gq12_w = (const double*)sec("gq12_w");
+    return 0;
+}
+
+extern "C" int host_transport(const pumas_gpu_context *ctx, const pumas_gpu_geometry_desc *d,
+    pumas_state *states, int64_t n, uint64_t seed, uint64_t first,
+    int *events, int *media, uint32_t *steps, uint32_t *draws, int *rcs) {
+    pg_geometry g; memset(&g, 0, sizeof(g));
+    std::vector<int> mat(d->n_media); std::vector<double> den(d->n_media);
+    for (int i = 0; i < d->n_media; i++) { mat[i] = d->media[i].material; den[i] = d->media[i].density; }
+    g.kind = d->kind; g.step_type = d->step_type; g.n_media = d->n_media; g.material = mat.data(); g.density = den.data();
+    g.n_layers = d->n_layers; g.z = d->z; g.nx = d->nx; g.ny = d->ny; g.nz = d->nz; g.x0 = d->x0; g.y0 = d->y0;
+    g.z0 = d->z0; g.dx = d->dx; g.dy = d->dy; g.dz = d->dz; g.height = d->height; g.cell_density = d->density;
+    g.label = d->label; g.z_top = d->z_top;
+    Config c; c.energy_loss = ctx->mode.energy_loss; c.decay = ctx->mode.decay; c.direction = ctx->mode.direction;
+    c.scattering = ctx->mode.scattering; c.event = ctx->event; c.limit_energy = ctx->limit.energy;
+    c.limit_distance = ctx->limit.distance; c.limit_grammage = ctx->limit.grammage; c.limit_time = ctx->limit.time;
+    c.accuracy = ctx->accuracy;
+    static FloatTables g_ft; Engine eng = {g_view, g_mv, c, g, g_ft};
+    for (int64_t i = 0; i < n; i++) {
+        pumas_state *s = states + i; Walker w; Track &t = w.t;
+        t.charge = s->charge; t.energy = s->energy; t.distance = s->distance; t.grammage = s->grammage;
+        t.time = s->time; t.weight = s->weight;
+        for (int k = 0; k < 3; k++) { t.pos[k] = s->position[k]; t.dir[k] = s->direction[k]; }
+        t.decayed = s->decayed; t.step_first = 0; t.step_event = t.step_foreseen = 0;
+        t.step_X_limit = t.step_invlb1 = t.lifetime = 0.; t.randn_done = 0; t.randn_next = 0.;
+        t.ix.reset(g_view.n_energies); t.fx.reset(g_view.n_energies); t.rng.init(seed, first + i, STREAM_TRANSPORT, 0); t.steps = 0; t.max_steps = 10000000;
+        eng.walker_run(w);
+        s->charge = t.charge; s->energy = t.energy; s->distance = t.distance; s->grammage = t.grammage;
+        s->time = t.time; s->weight = t.weight;
+        for (int k = 0; k < 3; k++) { s->position[k] = t.pos[k]; s->direction[k] = t.dir[k]; }
+        s->decayed = t.decayed;
+        events[i] = w.event; media[2*i] = w.media0; media[2*i+1] = w.end_medium; steps[i] = t.steps; draws[i] = t.rng.count; rcs[i] = w.rc;
+    }
+    return 0;
+}

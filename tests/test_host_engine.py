@@ -1,0 +1,118 @@
+"""The device engine compiled for the host, bit for bit against the oracle.
+
+tests/host_engine.cpp builds the transport engine headers of the library
+with g++ (no GPU, glibc libm) and runs each history with the same Philox
+stream as the oracle. With the libm of the oracle, every operation of the
+engine must reproduce the reference exactly: final states, events, media,
+step and draw counts are compared for equality on the golden cases and on
+terrain and voxel geometries. The comparison is with the oracle build whose
+float logf/expf are rounded double precision evaluations, the definition the
+engine uses on every platform (dcs_evaluate, src/pumas.c:11243-11347); the
+golden fixtures themselves come from the unmodified reference.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import pumas_amd as pa
+from pumas_amd import _abi
+from pumas_amd import workloads as W
+
+import make_golden
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(BUILD, "libhost_engine.so")
+
+
+@pytest.fixture(scope="module")
+def host():
+    os.makedirs(BUILD, exist_ok=True)
+    src = os.path.join(HERE, "host_engine.cpp")
+    csrc = os.path.join(REPO, "pumas_amd", "csrc")
+    deps = [src] + [os.path.join(csrc, f) for f in os.listdir(csrc)
+                    if f.endswith(".h")]
+    if (not os.path.exists(LIB)) or any(
+            os.path.getmtime(d) > os.path.getmtime(LIB) for d in deps):
+        subprocess.check_call(
+            ["g++", "-O2", "-std=c++17", "-fPIC", "-shared",
+             "-ffp-contract=off", "-I", csrc, "-I",
+             os.path.join(REPO, "include"), "-o", LIB, src])
+    lib = ctypes.CDLL(LIB)
+    lib.host_load.argtypes = [ctypes.c_char_p]
+    lib.host_transport.argtypes = [
+        ctypes.POINTER(_abi.GpuContext), ctypes.POINTER(_abi.GeometryDesc),
+        ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64,
+        ctypes.c_uint64] + [ctypes.c_void_p] * 5
+    return lib
+
+
+def run_host(lib, physics, ctx, desc, states, seed=0):
+    assert lib.host_load(pa.physics_path(physics).encode()) == 0
+    s = states.copy()
+    n = len(s)
+    ev = np.zeros(n, np.int32)
+    md = np.zeros((n, 2), np.int32)
+    st = np.zeros(n, np.uint32)
+    dr = np.zeros(n, np.uint32)
+    rc = np.zeros(n, np.int32)
+    c = ctx.to_c()
+    d = desc.to_c()
+    lib.host_transport(ctypes.byref(c), ctypes.byref(d), s.ctypes.data, n,
+                       seed, 0, ev.ctypes.data, md.ctypes.data,
+                       st.ctypes.data, dr.ctypes.data, rc.ctypes.data)
+    return s, ev, md, st, dr, rc
+
+
+def check_exact(host_out, ref, skip_aborted=False):
+    s, ev, md, st, dr, rc = host_out
+    ok = (ref.rc != -1) if skip_aborted else np.ones(len(s), bool)
+    assert ok.mean() > 0.5
+    for f in ("energy", "distance", "grammage", "time", "weight", "position",
+              "direction", "decayed", "charge"):
+        np.testing.assert_array_equal(s[f][ok], ref.states[f][ok], err_msg=f)
+    np.testing.assert_array_equal(ev[ok], ref.events[ok])
+    np.testing.assert_array_equal(md[ok], ref.media[ok])
+    np.testing.assert_array_equal(st[ok], ref.steps[ok])
+    np.testing.assert_array_equal(dr[ok], ref.draws[ok])
+
+
+@pytest.mark.parametrize("case", [c[0] for c in make_golden.cases()])
+def test_golden_cases_bit_exact(host, ref_standard_aligned, case):
+    import oracle as orc
+    name, pname, ctx, desc, states = next(
+        c for c in make_golden.cases() if c[0] == case)
+    states = states[:1024]
+    out = run_host(host, pname, ctx, desc, states)
+    ref = orc.transport(ref_standard_aligned, ctx,
+                        orc.RefGeometry(desc, aligned=True), states, seed=0,
+                        threads=8)
+    check_exact(out, ref)
+
+
+def test_terrain_bit_exact(host, ref_standard_aligned):
+    import oracle as orc
+    desc = W.terrain_geometry(64, spacing=20.0)
+    states = W.terrain_states(300, 63 * 20.0)
+    ctx = W.terrain_context()
+    out = run_host(host, "standard_c005", ctx, desc, states)
+    ref = orc.transport(ref_standard_aligned, ctx,
+                        orc.RefGeometry(desc, aligned=True), states, seed=0,
+                        threads=8)
+    check_exact(out, ref, skip_aborted=True)
+
+
+def test_voxels_bit_exact(host, ref_iron_aligned):
+    import oracle as orc
+    desc = W.voxel_geometry(32, size=8.0)
+    states = W.voxel_states(200, 32 * 8.0)
+    ctx = W.voxel_context()
+    out = run_host(host, "iron_c001", ctx, desc, states)
+    ref = orc.transport(ref_iron_aligned, ctx,
+                        orc.RefGeometry(desc, aligned=True), states, seed=0,
+                        threads=8)
+    check_exact(out, ref, skip_aborted=True)
