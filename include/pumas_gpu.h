@@ -192,7 +192,7 @@ enum pumas_gpu_geometry_kind {
         /* Horizontal layers: medium i fills z in [z[i], z[i+1]). */
         PUMAS_GPU_GEOMETRY_SLABS = 1,
         /* Heightmap terrain: medium 0 is the air above the ground, medium
-         * 1 + (iy * nx + ix) the rock column of cell (ix, iy). */
+         * 1 + (iy * (nx - 1) + ix) the rock column of cell (ix, iy). */
         PUMAS_GPU_GEOMETRY_TERRAIN = 2,
         /* Regular voxel grid: medium 1 + voxel index; material per voxel. */
         PUMAS_GPU_GEOMETRY_VOXELS = 3
@@ -221,7 +221,7 @@ struct pumas_gpu_geometry_desc {
         double x0, y0, z0;       /* grid origin, in m */
         double dx, dy, dz;       /* cell sizes, in m */
         const float * height;    /* TERRAIN: [ny][nx] node heights, in m */
-        const float * density;   /* TERRAIN: [ny][nx] column densities;
+        const float * density;   /* TERRAIN: [ny-1][nx-1] cell densities;
                                     VOXELS: [nz][ny][nx] densities, kg/m^3 */
         const uint8_t * label;   /* VOXELS: [nz][ny][nx] medium class */
         double z_top;            /* TERRAIN: top of the air layer, in m */
