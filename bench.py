@@ -52,6 +52,10 @@ def parse():
                     help="fp64: reference arithmetic; fp32: FP32-bulk")
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--serve-seconds", type=float, default=0.25,
+                    help="arrival window of the serving run (Poisson "
+                         "requests at 200/s); 0 skips it")
+    ap.add_argument("--serve-slots", type=int, default=0)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this "
                          "many MiB (calibrates HBM byte counters)")
@@ -98,6 +102,62 @@ def cpu_baseline(states, seconds):
         "kind": "reference",
         "sample": f"{n} histories of the same workload ({r.total_steps} "
                   f"steps, {r.seconds:.1f} s)",
+    }
+
+
+def serving(pa, phys, seconds, slots):
+    """Serving workload: Poisson arrivals (200/s) over `seconds` of
+    requests of 10^U[3,6] muons, forward slabs or backward terrain
+    (1024 x 1024 heightmap at 5 m), CSDA or MIXED energy loss, submitted to
+    the continuous batching server as they arrive. Latency = submission to
+    completion of a request, host buffers included."""
+    import threading
+    import numpy as np
+    from pumas_amd import workloads as W
+    reqs = W.serving_requests(seconds)
+    if not reqs:
+        return None
+    log(f"serving: {len(reqs)} requests, {sum(r.n for r in reqs)} muons")
+    geoms = {"slabs": pa.Geometry(phys, W.slabs_geometry()),
+             "terrain": pa.Geometry(phys, W.terrain_geometry())}
+    extent = 1023 * 5.0
+    inputs = [W.serving_inputs(r, extent) for r in reqs]
+    srv = pa.Server(slots=slots)
+    tickets = [None] * len(reqs)
+    ready = threading.Semaphore(0)
+    t0 = time.perf_counter()
+
+    def submitter():
+        for k, (r, (ctx, st)) in enumerate(zip(reqs, inputs)):
+            dt = r.arrival - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            tickets[k] = srv.submit(phys, ctx, geoms[r.geometry], st,
+                                    first_history=r.first, outputs=False)
+            ready.release()
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    out = []
+    for k in range(len(reqs)):
+        ready.acquire()
+        out.append(srv.wait(tickets[k]))
+    th.join()
+    wall = time.perf_counter() - t0
+    srv.close()
+    lat = np.array([o.latency_ms for o in out])
+    queue = np.array([o.queue_ms for o in out])
+    muons = sum(r.n for r in reqs)
+    steps = sum(o.steps for o in out)
+    return {
+        "requests": len(reqs), "muons": muons, "window_s": seconds,
+        "rate_per_s": 200.0, "slots": slots or 4,
+        "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)),
+        "mean_queue_ms": float(queue.mean()),
+        "wall_s": wall, "events_per_s": muons / wall,
+        "steps_per_s": steps / wall,
+        "note": "offered load above capacity: latency includes queueing",
     }
 
 
@@ -172,6 +232,11 @@ def main():
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
 
+    serve = None
+    if args.serve_seconds > 0:
+        log("bench: serving run")
+        serve = serving(pa, phys, args.serve_seconds, args.serve_slots)
+
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline (reference, all host cores)")
@@ -214,6 +279,7 @@ def main():
         },
         "transport_ms_per_pass": mean_ms,
         "cpu_baseline": cpu,
+        "serving": serve,
     }
     print(json.dumps(line))
 

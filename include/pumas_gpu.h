@@ -310,6 +310,140 @@ enum pumas_return pumas_gpu_transport_host(const pumas_gpu_physics * physics,
     const struct pumas_gpu_outputs * outputs,
     const struct pumas_gpu_options * options, struct pumas_gpu_stats * stats);
 
+/* ------------------------------------------------------------------------
+ * Tallies: flux estimators of transported histories, on the device.
+ *
+ * No counterpart in the reference library: its examples compute the same
+ * estimator in a host loop over histories, sum of weight x flux model
+ * (examples/pumas/straight.c:163-179 with examples/pumas/flux.c:39-89).
+ * ------------------------------------------------------------------------ */
+enum pumas_gpu_flux {
+        /* Weights only (flux factor 1). */
+        PUMAS_GPU_FLUX_NONE = 0,
+        /* Gaisser's model, per m^2 s sr GeV (examples/pumas/flux.c:57-65). */
+        PUMAS_GPU_FLUX_GAISSER = 1,
+        /* GCCLY: Gaisser with Volkova's cos(theta*) and the low energy
+         * correction of Guan et al. (examples/pumas/flux.c:70-89). */
+        PUMAS_GPU_FLUX_GCCLY = 2
+};
+
+#define PUMAS_GPU_TALLY_MAX_BINS 256
+
+struct pumas_gpu_tally_spec {
+        enum pumas_gpu_flux flux;
+        /* Non zero: multiply by the fraction of the flux with the state's
+         * charge (flux.c:39-52); zero: the total flux of both charges. */
+        int charge_fraction;
+        /* Factor applied to the flux model (0 = 1), e.g. 1e-4 for a flux
+         * per cm^2 instead of per m^2. */
+        double scale;
+        /* Non zero: cos(zenith) = -direction[2] of the final state (the
+         * muon arrives downwards); zero: the fixed `cos_theta` below. */
+        int cos_from_direction;
+        double cos_theta;
+        /* Histories whose event has a bit of the mask are tallied; 0, or no
+         * events given, selects all. */
+        unsigned int event_mask;
+        /* Histograms: log bins of the final kinetic energy (GeV) and linear
+         * bins of cos(zenith); 0 bins disables a histogram. */
+        int n_energy_bins;
+        double energy_min, energy_max;
+        int n_cos_bins;
+        double cos_min, cos_max;
+};
+
+struct pumas_gpu_tally_result {
+        double sum;      /* sum over tallied histories of weight x flux */
+        double sum2;     /* sum of squares, for the standard error */
+        uint64_t count;  /* tallied histories */
+        /* Host arrays given by the caller (or NULL): [2 * n_bins] holding
+         * per bin the sum and the sum of squares. */
+        double * energy;
+        double * cos_theta;
+};
+
+/*
+ * Tally `n` device-resident states (and their events, or NULL) on
+ * `stream`. The sums are deterministic (fixed-order reductions). The call
+ * waits for its stream.
+ */
+enum pumas_return pumas_gpu_tally(const struct pumas_state * states,
+    const enum pumas_event * events, size_t n,
+    const struct pumas_gpu_tally_spec * spec,
+    struct pumas_gpu_tally_result * result, void * stream);
+
+/* Same, with host arrays. */
+enum pumas_return pumas_gpu_tally_host(const struct pumas_state * states,
+    const enum pumas_event * events, size_t n,
+    const struct pumas_gpu_tally_spec * spec,
+    struct pumas_gpu_tally_result * result);
+
+/* ------------------------------------------------------------------------
+ * Request server: continuous batching of transport requests.
+ *
+ * No counterpart in the reference library (one history per call on the
+ * calling thread, src/pumas.c:3114). For callers that serve many uneven
+ * requests: each request is admitted as soon as a slot (a HIP stream) is
+ * free and its device memory fits the HBM budget beside the requests in
+ * flight, and runs concurrently with them. A request's results are the
+ * same alone or in any mix.
+ * ------------------------------------------------------------------------ */
+typedef struct pumas_gpu_server pumas_gpu_server;
+
+#define PUMAS_GPU_SERVER_MAX_SLOTS 32
+
+struct pumas_gpu_server_options {
+        /* Device memory for requests in flight, in bytes (0 = 200 GiB,
+         * capped by the free memory at creation). */
+        size_t hbm_budget;
+        /* Requests in flight at once, one stream each (0 = 4). */
+        int slots;
+};
+
+struct pumas_gpu_request {
+        const pumas_gpu_physics * physics;
+        const pumas_gpu_geometry * geometry;
+        struct pumas_gpu_context context;
+        enum pumas_gpu_precision precision;
+        /* States, updated in place, and optional outputs: host memory, or
+         * device memory when `device` is non zero. The caller keeps them
+         * alive until pumas_gpu_server_wait returns for this request. */
+        struct pumas_state * states;
+        size_t n;
+        uint64_t seed;
+        uint64_t first_history;
+        struct pumas_gpu_outputs outputs;
+        int device;
+};
+
+struct pumas_gpu_request_status {
+        enum pumas_return rc;
+        double queue_ms;    /* submission to admission */
+        double latency_ms;  /* submission to completion */
+        double kernel_ms;   /* transport time on the device */
+        uint64_t steps;
+        char message[256];
+};
+
+enum pumas_return pumas_gpu_server_create(
+    const struct pumas_gpu_server_options * options,
+    pumas_gpu_server ** server);
+
+/* Completes the requests submitted, then releases the server. */
+void pumas_gpu_server_destroy(pumas_gpu_server ** server);
+
+/* Queue a request; returns its ticket. The request structure is copied. */
+enum pumas_return pumas_gpu_server_submit(pumas_gpu_server * server,
+    const struct pumas_gpu_request * request, uint64_t * ticket);
+
+/* 1 if the request is done, 0 if pending, -1 for an unknown ticket. */
+int pumas_gpu_server_poll(pumas_gpu_server * server, uint64_t ticket);
+
+/* Wait for a request, fill its status and release its ticket; returns the
+ * request's return code. */
+enum pumas_return pumas_gpu_server_wait(pumas_gpu_server * server,
+    uint64_t ticket, struct pumas_gpu_request_status * status);
+
 /* Message of the last error raised on the calling thread, or "". */
 const char * pumas_gpu_error_message(void);
 

@@ -177,3 +177,28 @@ def philox(ctr, key):
     o = (ctypes.c_uint32 * 4)()
     library().ref_philox4x32_10(c, k, o)
     return tuple(o)
+
+
+LIBRARY_FLUX = os.path.join(HERE, "_ref", "libflux_ref.so")
+
+
+def flux(kind, cos_theta, kinetic, charge):
+    """The reference's example flux models (examples/pumas/flux.c),
+    kind 1 = flux_gaisser, 2 = flux_gccly, evaluated element-wise."""
+    if "flux" not in _libs:
+        lib = ctypes.CDLL(LIBRARY_FLUX)
+        lib.ref_flux_batch.argtypes = [ctypes.c_int, ctypes.c_size_t] + \
+            [ctypes.c_void_p] * 4
+        lib.ref_flux_batch.restype = ctypes.c_int
+        _libs["flux"] = lib
+    c = np.ascontiguousarray(cos_theta, dtype=np.float64)
+    k = np.ascontiguousarray(kinetic, dtype=np.float64)
+    q = np.ascontiguousarray(charge, dtype=np.float64)
+    c, k, q = np.broadcast_arrays(c, k, q)
+    c, k, q = (np.ascontiguousarray(x) for x in (c, k, q))
+    out = np.empty(c.shape)
+    rc = _libs["flux"].ref_flux_batch(kind, c.size, c.ctypes.data,
+                                      k.ctypes.data, q.ctypes.data,
+                                      out.ctypes.data)
+    assert rc == 0
+    return out

@@ -18,13 +18,16 @@ import os
 import numpy as np
 
 from . import _abi
-from ._abi import (Event, GeometryKind, Mode, Precision, Return, Step,
+from ._abi import (Event, Flux, GeometryKind, Mode, Precision, Return, Step,
                    STATE_DTYPE)
 
 __all__ = [
-    "Context", "Event", "Geometry", "GeometryDescription", "GeometryKind", "Mode", "Physics",
-    "Precision", "PumasError", "Return", "STATE_DTYPE", "Step", "library",
-    "new_states", "physics_path", "transport",
+    "Context", "Event", "Flux", "Geometry", "GeometryDescription",
+    "GeometryKind", "Mode", "Physics", "Precision", "PumasError", "Return",
+    "STATE_DTYPE", "Server", "Step", "Tally", "TallySpec", "library",
+    "new_states",
+    "physics_path", "tally", "tally_device", "transport",
+    "transport_device",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -367,3 +370,195 @@ def transport_device(physics, context, geometry, states_ptr, n, seed=0,
     if stats:
         return TransportStats.from_c(st)
     return None
+
+
+@dataclasses.dataclass
+class TallySpec:
+    """struct pumas_gpu_tally_spec: flux model, selection and histograms."""
+    flux: int = Flux.GAISSER
+    charge_fraction: bool = True
+    scale: float = 1.0
+    cos_from_direction: bool = True
+    cos_theta: float = 1.0
+    event_mask: int = 0
+    n_energy_bins: int = 0
+    energy_min: float = 1e-2
+    energy_max: float = 1e4
+    n_cos_bins: int = 0
+    cos_min: float = 0.0
+    cos_max: float = 1.0
+
+    def to_c(self):
+        c = _abi.TallySpec()
+        c.flux = int(self.flux)
+        c.charge_fraction = int(bool(self.charge_fraction))
+        c.scale = self.scale
+        c.cos_from_direction = int(bool(self.cos_from_direction))
+        c.cos_theta = self.cos_theta
+        c.event_mask = int(self.event_mask)
+        c.n_energy_bins = self.n_energy_bins
+        c.energy_min = self.energy_min
+        c.energy_max = self.energy_max
+        c.n_cos_bins = self.n_cos_bins
+        c.cos_min = self.cos_min
+        c.cos_max = self.cos_max
+        return c
+
+
+@dataclasses.dataclass
+class Tally:
+    """Flux estimator of a batch: mean and standard error of weight x flux
+    over the tallied histories, and the energy / cos(zenith) histograms
+    (per bin: sum, sum of squares)."""
+    sum: float
+    sum2: float
+    count: int
+    energy: object
+    cos_theta: object
+    n: int
+
+    @property
+    def mean(self):
+        """Estimator per history of the batch (as examples/pumas/straight.c:
+        every history counts in the denominator)."""
+        return self.sum / self.n if self.n else 0.0
+
+    @property
+    def error(self):
+        if self.n < 2:
+            return 0.0
+        m = self.mean
+        return np.sqrt(max(self.sum2 / self.n - m * m, 0.0) / self.n)
+
+
+def _tally_call(fn, states_arg, events_arg, n, spec):
+    c = spec.to_c()
+    r = _abi.TallyResult()
+    energy = np.zeros((max(spec.n_energy_bins, 0), 2))
+    cos_t = np.zeros((max(spec.n_cos_bins, 0), 2))
+    r.energy = energy.ctypes.data if spec.n_energy_bins > 0 else None
+    r.cos_theta = cos_t.ctypes.data if spec.n_cos_bins > 0 else None
+    _check(fn(states_arg, events_arg, n, ctypes.byref(c), ctypes.byref(r)))
+    return Tally(float(r.sum), float(r.sum2), int(r.count), energy, cos_t, n)
+
+
+def tally(states, events=None, spec=None):
+    """Tally host `states` (and `events`) on the device."""
+    lib = library()
+    spec = spec or TallySpec()
+    states = np.ascontiguousarray(states, dtype=STATE_DTYPE)
+    ev = None
+    if events is not None:
+        events = np.ascontiguousarray(events, dtype=np.int32)
+        ev = events.ctypes.data
+    return _tally_call(lib.pumas_gpu_tally_host, states.ctypes.data, ev,
+                       len(states), spec)
+
+
+def tally_device(states_ptr, n, events_ptr=None, spec=None, stream=None):
+    """Tally `n` device-resident states at `states_ptr` on `stream`."""
+    lib = library()
+    spec = spec or TallySpec()
+
+    def fn(s, e, n_, c, r):
+        return lib.pumas_gpu_tally(s, e, n_, c, r,
+                                   ctypes.c_void_p(stream) if stream else None)
+    return _tally_call(fn, ctypes.c_void_p(states_ptr),
+                       ctypes.c_void_p(events_ptr) if events_ptr else None,
+                       n, spec)
+
+
+@dataclasses.dataclass
+class RequestResult:
+    """A served request: its TransportResult and timings."""
+    result: TransportResult
+    rc: int
+    queue_ms: float
+    latency_ms: float
+    kernel_ms: float
+    steps: int
+    message: str
+
+
+class Server:
+    """Continuous batching server (pumas_gpu_server): requests run as soon
+    as a slot and their device memory are free, concurrently with the
+    requests in flight; results are those of the same request alone."""
+
+    def __init__(self, hbm_budget=0, slots=0):
+        lib = library()
+        o = _abi.ServerOptions()
+        o.hbm_budget = int(hbm_budget)
+        o.slots = int(slots)
+        h = ctypes.c_void_p()
+        _check(lib.pumas_gpu_server_create(ctypes.byref(o), ctypes.byref(h)))
+        self.handle = h
+        self._inflight = {}
+
+    def submit(self, physics, context, geometry, states, seed=0,
+               first_history=0, precision=Precision.FP64, outputs=True):
+        """Queue host `states` (updated in place); returns a ticket."""
+        lib = library()
+        states = np.ascontiguousarray(states, dtype=STATE_DTYPE)
+        n = len(states)
+        res = TransportResult(states=states)
+        r = _abi.Request()
+        r.physics = physics.handle
+        r.geometry = geometry.handle
+        r.context = context.to_c()
+        r.precision = int(precision)
+        r.states = states.ctypes.data
+        r.n = n
+        r.seed = seed
+        r.first_history = first_history
+        if outputs:
+            res.events = np.zeros(n, np.int32)
+            res.media = np.zeros((n, 2), np.int32)
+            res.steps = np.zeros(n, np.uint32)
+            res.draws = np.zeros(n, np.uint32)
+            res.rc = np.zeros(n, np.int32)
+            r.outputs.events = res.events.ctypes.data
+            r.outputs.media = res.media.ctypes.data
+            r.outputs.steps = res.steps.ctypes.data
+            r.outputs.draws = res.draws.ctypes.data
+            r.outputs.rc = res.rc.ctypes.data
+        r.device = 0
+        t = ctypes.c_uint64()
+        _check(lib.pumas_gpu_server_submit(self.handle, ctypes.byref(r),
+                                           ctypes.byref(t)))
+        # The arrays stay referenced until the request is waited for.
+        self._inflight[t.value] = (res, physics, geometry)
+        return t.value
+
+    def done(self, ticket):
+        return library().pumas_gpu_server_poll(self.handle, ticket) == 1
+
+    def wait(self, ticket, raise_errors=True):
+        lib = library()
+        st = _abi.RequestStatus()
+        rc = lib.pumas_gpu_server_wait(self.handle, ticket, ctypes.byref(st))
+        res = self._inflight.pop(ticket)[0]
+        if raise_errors:
+            _check(rc)
+        res.total_steps = int(st.steps)
+        res.kernel_ms = float(st.kernel_ms)
+        return RequestResult(res, int(st.rc), float(st.queue_ms),
+                             float(st.latency_ms), float(st.kernel_ms),
+                             int(st.steps), st.message.decode(errors="replace"))
+
+    def close(self):
+        if self.handle:
+            library().pumas_gpu_server_destroy(ctypes.byref(self.handle))
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -158,6 +158,57 @@ class GpuStats(ctypes.Structure):
 
 
 # Every entry point declared by include/pumas_gpu.h.
+class Flux(enum.IntEnum):
+    """enum pumas_gpu_flux."""
+    NONE = 0
+    GAISSER = 1
+    GCCLY = 2
+
+
+TALLY_MAX_BINS = 256
+
+
+class TallySpec(ctypes.Structure):
+    _fields_ = [("flux", ctypes.c_int), ("charge_fraction", ctypes.c_int),
+                ("scale", ctypes.c_double),
+                ("cos_from_direction", ctypes.c_int),
+                ("cos_theta", ctypes.c_double),
+                ("event_mask", ctypes.c_uint),
+                ("n_energy_bins", ctypes.c_int),
+                ("energy_min", ctypes.c_double),
+                ("energy_max", ctypes.c_double),
+                ("n_cos_bins", ctypes.c_int),
+                ("cos_min", ctypes.c_double), ("cos_max", ctypes.c_double)]
+
+
+class TallyResult(ctypes.Structure):
+    _fields_ = [("sum", ctypes.c_double), ("sum2", ctypes.c_double),
+                ("count", ctypes.c_uint64), ("energy", ctypes.c_void_p),
+                ("cos_theta", ctypes.c_void_p)]
+
+
+SERVER_MAX_SLOTS = 32
+
+
+class ServerOptions(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_size_t), ("slots", ctypes.c_int)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("physics", ctypes.c_void_p), ("geometry", ctypes.c_void_p),
+                ("context", GpuContext), ("precision", ctypes.c_int),
+                ("states", ctypes.c_void_p), ("n", ctypes.c_size_t),
+                ("seed", ctypes.c_uint64), ("first_history", ctypes.c_uint64),
+                ("outputs", GpuOutputs), ("device", ctypes.c_int)]
+
+
+class RequestStatus(ctypes.Structure):
+    _fields_ = [("rc", ctypes.c_int), ("queue_ms", ctypes.c_double),
+                ("latency_ms", ctypes.c_double),
+                ("kernel_ms", ctypes.c_double), ("steps", ctypes.c_uint64),
+                ("message", ctypes.c_char * 256)]
+
+
 EXPORTED_SYMBOLS = (
     "pumas_gpu_context_default",
     "pumas_gpu_physics_upload",
@@ -170,6 +221,13 @@ EXPORTED_SYMBOLS = (
     "pumas_gpu_geometry_destroy",
     "pumas_gpu_transport",
     "pumas_gpu_transport_host",
+    "pumas_gpu_tally",
+    "pumas_gpu_tally_host",
+    "pumas_gpu_server_create",
+    "pumas_gpu_server_destroy",
+    "pumas_gpu_server_submit",
+    "pumas_gpu_server_poll",
+    "pumas_gpu_server_wait",
     "pumas_gpu_error_message",
     "pumas_gpu_version",
 )
@@ -205,6 +263,24 @@ def declare(lib):
         vp, P(GpuContext), vp, vp, ctypes.c_size_t, ctypes.c_uint64,
         ctypes.c_uint64, P(GpuOutputs), P(GpuOptions), P(GpuStats)]
     lib.pumas_gpu_transport_host.restype = ctypes.c_int
+    lib.pumas_gpu_tally.argtypes = [vp, vp, ctypes.c_size_t, P(TallySpec),
+                                    P(TallyResult), vp]
+    lib.pumas_gpu_tally.restype = ctypes.c_int
+    lib.pumas_gpu_tally_host.argtypes = [vp, vp, ctypes.c_size_t,
+                                         P(TallySpec), P(TallyResult)]
+    lib.pumas_gpu_tally_host.restype = ctypes.c_int
+    lib.pumas_gpu_server_create.argtypes = [P(ServerOptions), P(vp)]
+    lib.pumas_gpu_server_create.restype = ctypes.c_int
+    lib.pumas_gpu_server_destroy.argtypes = [P(vp)]
+    lib.pumas_gpu_server_destroy.restype = None
+    lib.pumas_gpu_server_submit.argtypes = [vp, P(Request),
+                                            P(ctypes.c_uint64)]
+    lib.pumas_gpu_server_submit.restype = ctypes.c_int
+    lib.pumas_gpu_server_poll.argtypes = [vp, ctypes.c_uint64]
+    lib.pumas_gpu_server_poll.restype = ctypes.c_int
+    lib.pumas_gpu_server_wait.argtypes = [vp, ctypes.c_uint64,
+                                          P(RequestStatus)]
+    lib.pumas_gpu_server_wait.restype = ctypes.c_int
     lib.pumas_gpu_error_message.argtypes = []
     lib.pumas_gpu_error_message.restype = ctypes.c_char_p
     lib.pumas_gpu_version.argtypes = []

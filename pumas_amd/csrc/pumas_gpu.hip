@@ -1,12 +1,6 @@
 /*
- * libpumas_hip: C ABI, table upload and transport kernels for gfx950.
- *
- * One work item transports one history at a time. The persistent kernel
- * keeps a fixed number of wavefronts resident and refills lanes whose
- * history ended from a global queue, one atomic per wavefront (ballot of
- * the free lanes and a prefix count to hand out consecutive history ids),
- * so that uneven history lengths do not leave lanes idle until the longest
- * history of a wavefront ends.
+ * libpumas_hip: C ABI, table upload, geometry upload and the transport
+ * kernels for gfx950 (a wavefront scheduler over per-phase queues, below).
  */
 #include <hip/hip_runtime.h>
 
@@ -20,6 +14,7 @@
 #include "../../include/pumas_gpu.h"
 #include "engine_transport.h"
 #include "ptab.h"
+#include "status.h"
 
 using namespace pg;
 
@@ -28,7 +23,7 @@ using namespace pg;
  * ------------------------------------------------------------------------ */
 static thread_local char g_error[1024] = "";
 
-static enum pumas_return fail(enum pumas_return rc, const char * fmt, ...)
+enum pumas_return pg_fail(enum pumas_return rc, const char * fmt, ...)
 {
         va_list ap;
         va_start(ap, fmt);
@@ -37,14 +32,7 @@ static enum pumas_return fail(enum pumas_return rc, const char * fmt, ...)
         return rc;
 }
 
-#define HIP_TRY(expr)                                                          \
-        do {                                                                   \
-                hipError_t e_ = (expr);                                        \
-                if (e_ != hipSuccess)                                          \
-                        return fail(PUMAS_RETURN_INTERNAL_ERROR,               \
-                            "%s: %s (%s:%d)", #expr, hipGetErrorString(e_),    \
-                            __FILE__, __LINE__);                               \
-        } while (0)
+void pg_clear_error(void) { g_error[0] = '\0'; }
 
 extern "C" const char * pumas_gpu_error_message(void) { return g_error; }
 
@@ -87,23 +75,23 @@ enum pumas_return parse_ptab(const uint8_t * image, size_t size,
     ptab_header * header, std::vector<Section> & sections)
 {
         if (size < sizeof(ptab_header))
-                return fail(PUMAS_RETURN_FORMAT_ERROR, "PTAB image too small");
+                return pg_fail(PUMAS_RETURN_FORMAT_ERROR, "PTAB image too small");
         memcpy(header, image, sizeof(*header));
         if (memcmp(header->magic, PTAB_MAGIC, 8) != 0)
-                return fail(PUMAS_RETURN_FORMAT_ERROR, "not a PTAB image");
+                return pg_fail(PUMAS_RETURN_FORMAT_ERROR, "not a PTAB image");
         if (header->version != PTAB_VERSION)
-                return fail(PUMAS_RETURN_FORMAT_ERROR,
+                return pg_fail(PUMAS_RETURN_FORMAT_ERROR,
                     "unsupported PTAB version %u", header->version);
         const size_t dir_end = sizeof(ptab_header) +
             (size_t)header->n_entries * sizeof(ptab_entry);
         if (dir_end > size)
-                return fail(PUMAS_RETURN_INCOMPLETE_FILE, "truncated PTAB");
+                return pg_fail(PUMAS_RETURN_INCOMPLETE_FILE, "truncated PTAB");
         const ptab_entry * e =
             reinterpret_cast<const ptab_entry *>(image + sizeof(ptab_header));
         for (uint32_t i = 0; i < header->n_entries; i++) {
                 const size_t width = (e[i].dtype == PTAB_F64) ? 8 : 4;
                 if (e[i].offset + e[i].count * width > size)
-                        return fail(PUMAS_RETURN_INCOMPLETE_FILE,
+                        return pg_fail(PUMAS_RETURN_INCOMPLETE_FILE,
                             "truncated PTAB section %s", e[i].name);
                 char name[PTAB_NAME_LEN + 1] = { 0 };
                 memcpy(name, e[i].name, PTAB_NAME_LEN);
@@ -125,7 +113,7 @@ extern "C" enum pumas_return pumas_gpu_physics_upload_memory(
     const void * ptab, size_t size, pumas_gpu_physics ** physics)
 {
         if (physics == NULL)
-                return fail(PUMAS_RETURN_VALUE_ERROR, "physics is NULL");
+                return pg_fail(PUMAS_RETURN_VALUE_ERROR, "physics is NULL");
         *physics = NULL;
         const uint8_t * image = static_cast<const uint8_t *>(ptab);
         ptab_header h;
@@ -135,14 +123,14 @@ extern "C" enum pumas_return pumas_gpu_physics_upload_memory(
 
         const int nM = h.imeta[PTAB_N_MATERIALS];
         if (h.imeta[PTAB_MAX_COMPONENTS] > MAX_COMPONENTS)
-                return fail(PUMAS_RETURN_VALUE_ERROR,
+                return pg_fail(PUMAS_RETURN_VALUE_ERROR,
                     "materials with more than %d elements are not supported",
                     MAX_COMPONENTS);
 
         /* Derived section: first component of each material. */
         const Section * sin = find(sections, "elements_in");
         if (sin == nullptr)
-                return fail(PUMAS_RETURN_FORMAT_ERROR, "missing elements_in");
+                return pg_fail(PUMAS_RETURN_FORMAT_ERROR, "missing elements_in");
         std::vector<int32_t> component0(nM);
         {
                 const int32_t * ein =
@@ -165,7 +153,7 @@ extern "C" enum pumas_return pumas_gpu_physics_upload_memory(
         for (int k = 0; k < n_mirrored; k++) {
                 const Section * sec = find(sections, mirrored[k]);
                 if ((sec == nullptr) || (sec->dtype != PTAB_F64))
-                        return fail(PUMAS_RETURN_FORMAT_ERROR,
+                        return pg_fail(PUMAS_RETURN_FORMAT_ERROR,
                             "PTAB section `%s' is missing", mirrored[k]);
                 float_at[k] = floats.size();
                 const double * src =
@@ -191,7 +179,7 @@ extern "C" enum pumas_return pumas_gpu_physics_upload_memory(
             hipMemcpyHostToDevice);
         if ((e1 != hipSuccess) || (e2 != hipSuccess) || (e3 != hipSuccess)) {
                 (void)hipFree(device);
-                return fail(PUMAS_RETURN_INTERNAL_ERROR,
+                return pg_fail(PUMAS_RETURN_INTERNAL_ERROR,
                     "could not copy the tables to the device");
         }
 
@@ -308,7 +296,7 @@ extern "C" enum pumas_return pumas_gpu_physics_upload_memory(
         if (missing) {
                 (void)hipFree(device);
                 delete ph;
-                return fail(PUMAS_RETURN_FORMAT_ERROR,
+                return pg_fail(PUMAS_RETURN_FORMAT_ERROR,
                     "PTAB section `%s' is missing", missing_name.c_str());
         }
         *physics = ph;
@@ -321,7 +309,7 @@ extern "C" enum pumas_return pumas_gpu_physics_upload(
         if (physics != NULL) *physics = NULL;
         FILE * f = fopen(path, "rb");
         if (f == NULL)
-                return fail(PUMAS_RETURN_PATH_ERROR, "could not open `%s'", path);
+                return pg_fail(PUMAS_RETURN_PATH_ERROR, "could not open `%s'", path);
         fseek(f, 0, SEEK_END);
         const long size = ftell(f);
         fseek(f, 0, SEEK_SET);
@@ -329,7 +317,7 @@ extern "C" enum pumas_return pumas_gpu_physics_upload(
         const size_t n = fread(image.data(), 1, image.size(), f);
         fclose(f);
         if (n != image.size())
-                return fail(PUMAS_RETURN_IO_ERROR, "could not read `%s'", path);
+                return pg_fail(PUMAS_RETURN_IO_ERROR, "could not read `%s'", path);
         return pumas_gpu_physics_upload_memory(image.data(), image.size(),
             physics);
 }
@@ -393,16 +381,16 @@ extern "C" enum pumas_return pumas_gpu_geometry_create(
     const pumas_gpu_physics * physics,
     const struct pumas_gpu_geometry_desc * desc, pumas_gpu_geometry ** out)
 {
-        if (out == NULL) return fail(PUMAS_RETURN_VALUE_ERROR, "NULL output");
+        if (out == NULL) return pg_fail(PUMAS_RETURN_VALUE_ERROR, "NULL output");
         *out = NULL;
         if ((physics == NULL) || (desc == NULL))
-                return fail(PUMAS_RETURN_VALUE_ERROR, "NULL argument");
+                return pg_fail(PUMAS_RETURN_VALUE_ERROR, "NULL argument");
         if ((desc->n_media <= 0) || (desc->media == NULL))
-                return fail(PUMAS_RETURN_MEDIUM_ERROR, "no media");
+                return pg_fail(PUMAS_RETURN_MEDIUM_ERROR, "no media");
         for (int i = 0; i < desc->n_media; i++) {
                 const int m = desc->media[i].material;
                 if ((m < 0) || (m >= physics->view.n_materials))
-                        return fail(PUMAS_RETURN_INDEX_ERROR,
+                        return pg_fail(PUMAS_RETURN_INDEX_ERROR,
                             "invalid material index [%d] for medium %d", m, i);
         }
 
@@ -437,13 +425,13 @@ extern "C" enum pumas_return pumas_gpu_geometry_create(
                 if ((desc->n_layers <= 0) || (desc->z == NULL) ||
                     (desc->n_media < desc->n_layers)) {
                         pumas_gpu_geometry_destroy(&g);
-                        return fail(PUMAS_RETURN_VALUE_ERROR,
+                        return pg_fail(PUMAS_RETURN_VALUE_ERROR,
                             "slabs need n_layers + 1 boundaries and media");
                 }
                 for (int i = 0; i < desc->n_layers; i++)
                         if (!(desc->z[i] < desc->z[i + 1])) {
                                 pumas_gpu_geometry_destroy(&g);
-                                return fail(PUMAS_RETURN_VALUE_ERROR,
+                                return pg_fail(PUMAS_RETURN_VALUE_ERROR,
                                     "slab boundaries must increase");
                         }
                 v.n_layers = desc->n_layers;
@@ -454,7 +442,7 @@ extern "C" enum pumas_return pumas_gpu_geometry_create(
                     (desc->density == NULL) || (desc->n_media < 2) ||
                     (desc->dx <= 0.) || (desc->dy <= 0.)) {
                         pumas_gpu_geometry_destroy(&g);
-                        return fail(PUMAS_RETURN_VALUE_ERROR,
+                        return pg_fail(PUMAS_RETURN_VALUE_ERROR,
                             "invalid terrain description");
                 }
                 v.nx = desc->nx;
@@ -475,19 +463,19 @@ extern "C" enum pumas_return pumas_gpu_geometry_create(
                     (desc->label == NULL) || (desc->density == NULL) ||
                     (desc->dx <= 0.) || (desc->dy <= 0.) || (desc->dz <= 0.)) {
                         pumas_gpu_geometry_destroy(&g);
-                        return fail(PUMAS_RETURN_VALUE_ERROR,
+                        return pg_fail(PUMAS_RETURN_VALUE_ERROR,
                             "invalid voxel description");
                 }
                 const size_t nv = (size_t)desc->nx * desc->ny * desc->nz;
                 if (nv >= (size_t)0x7fffffff) {
                         pumas_gpu_geometry_destroy(&g);
-                        return fail(PUMAS_RETURN_VALUE_ERROR,
+                        return pg_fail(PUMAS_RETURN_VALUE_ERROR,
                             "too many voxels");
                 }
                 for (size_t i = 0; i < nv; i++)
                         if (desc->label[i] >= desc->n_media) {
                                 pumas_gpu_geometry_destroy(&g);
-                                return fail(PUMAS_RETURN_INDEX_ERROR,
+                                return pg_fail(PUMAS_RETURN_INDEX_ERROR,
                                     "voxel %zu has an invalid label", i);
                         }
                 v.nx = desc->nx;
@@ -505,7 +493,7 @@ extern "C" enum pumas_return pumas_gpu_geometry_create(
         }
         default:
                 pumas_gpu_geometry_destroy(&g);
-                return fail(PUMAS_RETURN_VALUE_ERROR, "unknown geometry kind");
+                return pg_fail(PUMAS_RETURN_VALUE_ERROR, "unknown geometry kind");
         }
 #undef UP
         *out = g;
@@ -894,7 +882,7 @@ enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
                     stream, a);
         if (hipGetLastError() != hipSuccess) {
                 (void)hipFreeAsync(mem, stream);
-                return fail(PUMAS_RETURN_VALUE_ERROR,
+                return pg_fail(PUMAS_RETURN_VALUE_ERROR,
                     "transport kernel launch failed");
         }
         int parity = 0, launched = 0;
@@ -934,20 +922,20 @@ enum pumas_return check_context(const pumas_gpu_physics * physics,
     const struct pumas_gpu_context * c)
 {
         if ((c->accuracy <= 0) || (c->accuracy > 1))
-                return fail(PUMAS_RETURN_ACCURACY_ERROR,
+                return pg_fail(PUMAS_RETURN_ACCURACY_ERROR,
                     "bad accuracy value (expected a value in ]0,1], got %g)",
                     c->accuracy);
         if ((c->mode.direction == PUMAS_MODE_BACKWARD) &&
             (c->mode.energy_loss > PUMAS_MODE_CSDA) &&
             (physics->view.cutoff < 1E-02))
-                return fail(PUMAS_RETURN_ACCURACY_ERROR,
+                return pg_fail(PUMAS_RETURN_ACCURACY_ERROR,
                     "bad cutoff value for backward transport (expected a "
                     "value greater than or equal to 0.01, got %g)",
                     physics->view.cutoff);
         if ((physics->particle != 0) &&
             (c->mode.direction == PUMAS_MODE_FORWARD) &&
             (c->mode.decay == PUMAS_MODE_WEIGHTED))
-                return fail(PUMAS_RETURN_DECAY_ERROR,
+                return pg_fail(PUMAS_RETURN_DECAY_ERROR,
                     "`PUMAS_MODE_WEIGHTED' mode is not valid for forward taus");
         return PUMAS_RETURN_SUCCESS;
 }
@@ -962,18 +950,18 @@ extern "C" enum pumas_return pumas_gpu_transport(
     const struct pumas_gpu_options * options, struct pumas_gpu_stats * stats,
     void * stream_)
 {
-        g_error[0] = '\0';
+        pg_clear_error();
         if ((physics == NULL) || (context == NULL) || (geometry == NULL))
-                return fail(PUMAS_RETURN_VALUE_ERROR, "NULL argument");
+                return pg_fail(PUMAS_RETURN_VALUE_ERROR, "NULL argument");
         if ((states == NULL) && (n > 0))
-                return fail(PUMAS_RETURN_VALUE_ERROR, "no states (null)");
+                return pg_fail(PUMAS_RETURN_VALUE_ERROR, "no states (null)");
         enum pumas_return rc = check_context(physics, context);
         if (rc != PUMAS_RETURN_SUCCESS) return rc;
         const int precision =
             options ? options->precision : PUMAS_GPU_PRECISION_FP64;
         if ((precision != PUMAS_GPU_PRECISION_FP64) &&
             (precision != PUMAS_GPU_PRECISION_FP32_BULK))
-                return fail(PUMAS_RETURN_VALUE_ERROR,
+                return pg_fail(PUMAS_RETURN_VALUE_ERROR,
                     "unknown precision mode %d", precision);
         hipStream_t stream = static_cast<hipStream_t>(stream_);
 
@@ -1016,7 +1004,7 @@ extern "C" enum pumas_return pumas_gpu_transport(
             options->block_size :
             256;
         if ((block % 64) || (block > 256))
-                return fail(PUMAS_RETURN_VALUE_ERROR,
+                return pg_fail(PUMAS_RETURN_VALUE_ERROR,
                     "block size must be a multiple of 64, at most 256");
         const int per_cu = (options && options->blocks_per_cu > 0) ?
             options->blocks_per_cu :
@@ -1026,7 +1014,7 @@ extern "C" enum pumas_return pumas_gpu_transport(
             options->rounds_per_check :
             ROUNDS_PER_CHECK;
         if (rounds_per_check > MAX_ROUNDS_PER_CHECK)
-                return fail(PUMAS_RETURN_VALUE_ERROR,
+                return pg_fail(PUMAS_RETURN_VALUE_ERROR,
                     "rounds_per_check must be at most %d",
                     MAX_ROUNDS_PER_CHECK);
 
@@ -1066,7 +1054,7 @@ extern "C" enum pumas_return pumas_gpu_transport(
         }
         if (rc != PUMAS_RETURN_SUCCESS) return rc;
         if (error != 0)
-                return fail((enum pumas_return)error,
+                return pg_fail((enum pumas_return)error,
                     "a history raised error code %llu", error);
         return PUMAS_RETURN_SUCCESS;
 }

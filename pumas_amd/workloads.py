@@ -14,6 +14,7 @@ Workload sizes follow the configurations the library is benchmarked on:
   - `slabs_forward`: forward straggled transport with multiple scattering
     through 100 m of air, 500 m of standard rock and 200 m of water.
 """
+import dataclasses
 import math
 
 import numpy as np
@@ -347,3 +348,48 @@ def voxel_context():
     return Context(energy_loss=Mode.STRAGGLED, decay=Mode.WEIGHTED,
                    direction=Mode.BACKWARD, scattering=Mode.MIXED,
                    event=Event.NONE)
+
+
+# ---------------------------------------------------------------------------
+# Serving: Poisson arrivals of transport requests of uneven sizes.
+# ---------------------------------------------------------------------------
+@dataclasses.dataclass
+class ServingRequest:
+    arrival: float      # s after the start of the run
+    geometry: str       # "slabs" (forward) or "terrain" (backward)
+    energy_loss: int    # Mode.CSDA or Mode.MIXED
+    n: int              # muons
+    first: int          # first history id (Philox counter)
+
+
+def serving_requests(duration, rate=200.0, seed=0, nmin=1e3, nmax=1e6):
+    """Requests arriving over `duration` s at `rate` per s (exponential
+    gaps), each with round(10^U[log nmin, log nmax]) muons, geometry slabs
+    or terrain and energy loss CSDA or MIXED with probability 1/2 each.
+    History ids are disjoint across requests."""
+    out, t, k, first = [], 0.0, 0, 0
+    while True:
+        u = uniform(seed, 4, np.array([k], np.uint64), 0)[0]
+        t += -math.log(u) / rate
+        if t >= duration:
+            break
+        v = [uniform(seed, 4, np.array([k], np.uint64), j)[0]
+             for j in (1, 2, 3)]
+        n = int(round(10 ** (math.log10(nmin) +
+                            (math.log10(nmax) - math.log10(nmin)) * v[0])))
+        out.append(ServingRequest(
+            t, "slabs" if v[1] < 0.5 else "terrain",
+            Mode.CSDA if v[2] < 0.5 else Mode.MIXED, n, first))
+        first += n
+        k += 1
+    return out
+
+
+def serving_inputs(req, terrain_extent, seed=0):
+    """(context, initial states) of a serving request."""
+    if req.geometry == "slabs":
+        return (slabs_forward_context(energy_loss=req.energy_loss),
+                slabs_forward_states(req.n, seed=seed, first=req.first))
+    ctx = terrain_context().replace(energy_loss=req.energy_loss)
+    return ctx, terrain_states(req.n, terrain_extent, seed=seed,
+                               first=req.first)

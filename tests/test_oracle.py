@@ -209,3 +209,28 @@ def test_oracle_reproduces_golden(ref_standard, name):
     assert (r.states == g["final"]).all()
     for k in ("events", "media", "steps", "draws", "rc"):
         assert (getattr(r, k) == g[k]).all(), k
+
+
+def test_flux_models_invariants():
+    """The reference's example flux models (compiled from
+    examples/pumas/flux.c): the charge fractions of both signs add up to the
+    total flux; Volkova's cos(theta*) is 1 at the vertical, so that GCCLY
+    tends to Gaisser at high energy; the Gaisser flux falls as E^-2.7 far
+    below the pion and kaon critical energies' corrections."""
+    import oracle as orc
+    if not os.path.exists(orc.LIBRARY_FLUX):
+        pytest.skip("oracle not built")
+    k = np.logspace(-2, 4, 25)
+    c = np.linspace(0.1, 1.0, 25)
+    for kind in (1, 2):
+        fm = orc.flux(kind, c, k, -1.0)
+        fp = orc.flux(kind, c, k, 1.0)
+        f0 = orc.flux(kind, c, k, 0.0)
+        np.testing.assert_allclose(fm + fp, f0, rtol=1e-14)
+        np.testing.assert_allclose(fp / fm, 1.2766, rtol=1e-14)
+    kh = np.array([1e5, 1e6])
+    np.testing.assert_allclose(orc.flux(2, 1.0, kh, 0.0),
+                               orc.flux(1, 1.0, kh, 0.0), rtol=1e-3)
+    e = np.array([1e-1, 1e-2]) + 0.10566
+    f = orc.flux(1, 1.0, e - 0.10566, 0.0)
+    np.testing.assert_allclose(f[0] / f[1], (e[0] / e[1]) ** -2.7, rtol=2e-3)
