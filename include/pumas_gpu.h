@@ -272,7 +272,7 @@ struct pumas_gpu_stats {
         double kernel_ms;       /* transport kernel time (HIP events) */
         double step_kernel_ms;  /* time in the stepping kernel */
         uint64_t step_launches; /* launches of the stepping kernel */
-        uint64_t rounds;        /* scheduler rounds with work */
+        uint64_t rounds;        /* host checks of the queue lengths */
 };
 
 /*

@@ -506,20 +506,23 @@ extern "C" enum pumas_return pumas_gpu_geometry_create(
  * A batch runs in rounds. k_init loads every history and runs the entry
  * logic of pumas_context_transport; histories that need stepping get a
  * record (their Walker, in HBM) and a slot in the step queue. Each round
- * then runs three kernels:
+ * then runs two kernels:
  *   k_step  persistent; a lane takes a slot from the step queue and steps
- *           that history in registers until it ends or reaches a discrete
- *           event, then writes it back and appends the slot to the DEL or
- *           EHS queue (one atomic per wavefront and queue);
- *   k_ehs   one lane per queued elastic hard scattering;
- *   k_del   one lane per queued inelastic DEL;
- * both of which return their histories to the next round's step queue.
- * The step kernel holds only the stepping code, so its Walker stays in
- * registers without spills; the rarer, register- and scratch-hungry event
- * samplers run in their own kernels with every lane busy on the same code.
- * Each history still sees its own operations in the reference's order: only
- * the interleaving across histories differs, and the random stream of a
- * history is keyed by its id, so results do not depend on the schedule.
+ *           that history in registers, elastic hard scatterings (EHS)
+ *           included, until it ends or reaches an inelastic DEL vertex,
+ *           then writes it back and appends its slot to the DEL queue (one
+ *           atomic per wavefront); free lanes refill by ballot;
+ *   k_del   one lane per queued DEL, all lanes on the same sampling code;
+ *           the histories go back to the next round's step queue.
+ * EHS vertices (~16% of the steps of a muon in rock) are batched inside the
+ * stepping wavefront: lanes that reach one wait until enough others do.
+ * DEL vertices (~0.2% of the steps at a 5% cutoff) leave the wavefront: the
+ * DEL samplers are the largest code of the path (exact DCS models, polar
+ * angles, rejection loops) and would hold the stepping kernel's registers
+ * for every step. A history's operations are the reference's, in its
+ * order; only the interleaving across histories differs, and the random
+ * stream of a history is keyed by its id, so results do not depend on the
+ * schedule.
  * ------------------------------------------------------------------------ */
 enum {
         C_TAKEN = 0, /* step queue entries taken in this round */
@@ -527,7 +530,6 @@ enum {
         C_ERROR,     /* first error code raised by a history */
         C_STEP_N0,   /* step queue lengths (ping-pong) */
         C_STEP_N1,
-        C_EHS_N,     /* EHS queue length */
         C_DEL_N,     /* DEL queue length */
         C_COUNT = 8
 };
@@ -551,7 +553,6 @@ struct KernelArgs {
         unsigned long long * counters;
         Walker * records;     /* [n] histories in flight */
         uint32_t * q_step[2]; /* [n] step queues (ping-pong) */
-        uint32_t * q_ehs;     /* [n] */
         uint32_t * q_del;     /* [n] */
 };
 
@@ -675,12 +676,15 @@ __global__ void __launch_bounds__(256) k_init(KernelArgs args)
             w.phase == PHASE_STEP, (uint32_t)i);
 }
 
+/* Maximum number of lanes of a wavefront waiting for an elastic hard
+ * scattering before the wavefront processes them together. */
+constexpr int EHS_BATCH = 16;
+
 /* Clears the per-round counters before round `parity`. */
 __global__ void k_round(unsigned long long * c, int parity)
 {
         if (threadIdx.x == 0) {
                 c[C_TAKEN] = 0;
-                c[C_EHS_N] = 0;
                 c[C_DEL_N] = 0;
                 c[C_STEP_N0 + (parity ^ 1)] = 0;
         }
@@ -704,16 +708,14 @@ __global__ void __launch_bounds__(256, 2) k_step(KernelArgs args, int parity)
         for (;;) {
                 /* Hand over the histories that left the stepping phase. */
                 const bool done = w.phase == PHASE_DONE;
-                const bool to_ehs = w.phase == PHASE_EHS;
                 const bool to_del = w.phase == PHASE_DEL;
                 if (done) {
                         walker_store(a, slot, w);
                         steps_acc += w.t.steps;
                 }
-                if (to_ehs || to_del) a.records[slot] = w;
-                queue_append(a.counters + C_EHS_N, a.q_ehs, to_ehs, slot);
+                if (to_del) a.records[slot] = w;
                 queue_append(a.counters + C_DEL_N, a.q_del, to_del, slot);
-                if (done || to_ehs || to_del) w.phase = PHASE_EMPTY;
+                if (done || to_del) w.phase = PHASE_EMPTY;
 
                 /* Refill the free lanes from the step queue. */
                 if (!exhausted) {
@@ -739,25 +741,40 @@ __global__ void __launch_bounds__(256, 2) k_step(KernelArgs args, int parity)
                                 }
                         }
                 }
-                if (!__ballot(w.phase == PHASE_STEP)) {
+                if (!__ballot((w.phase == PHASE_STEP) ||
+                        (w.phase == PHASE_EHS))) {
                         if (exhausted) break;
                         continue;
                 }
                 if (w.phase == PHASE_STEP) eng.walker_step(w);
+
+                /* Elastic hard scatterings, on ~16% of the steps of a muon
+                 * in rock, run once enough lanes wait for one (half of the
+                 * lanes in flight, at most EHS_BATCH) or none can step, so
+                 * that the wavefront runs that code with most lanes busy. */
+                const uint64_t ehs = __ballot(w.phase == PHASE_EHS);
+                const uint64_t stepping = __ballot(w.phase == PHASE_STEP);
+                const int waiting = __popcll(ehs);
+                const int in_flight = waiting + __popcll(stepping);
+                const int batch = (in_flight + 1) / 2 < EHS_BATCH ?
+                    (in_flight + 1) / 2 :
+                    EHS_BATCH;
+                if (ehs && ((waiting >= batch) || !stepping)) {
+                        if (w.phase == PHASE_EHS) eng.walker_ehs(w);
+                }
         }
         steps_flush(a, steps_acc);
 }
 
-/* Discrete events of round `parity`, one lane per queued history; the
+/* Inelastic DELs of round `parity`, one lane per queued history; the
  * histories go back to the next round's step queue. */
-template <bool F32, int PHASE>
-__global__ void __launch_bounds__(256, 2) k_event(KernelArgs args, int parity)
+template <bool F32>
+__global__ void __launch_bounds__(256, 2) k_del(KernelArgs args, int parity)
 {
         KERNEL_ARGS(args);
         const EngineT<F32> eng = { a.p, a.mv, a.c, a.g, a.ft };
-        const unsigned long long n =
-            a.counters[(PHASE == PHASE_EHS) ? C_EHS_N : C_DEL_N];
-        const uint32_t * queue = (PHASE == PHASE_EHS) ? a.q_ehs : a.q_del;
+        const unsigned long long n = a.counters[C_DEL_N];
+        const uint32_t * queue = a.q_del;
         const int lane = threadIdx.x & 63;
         const unsigned long long stride = (unsigned long long)gridDim.x *
             blockDim.x;
@@ -773,10 +790,7 @@ __global__ void __launch_bounds__(256, 2) k_event(KernelArgs args, int parity)
                 if (active) {
                         slot = queue[k];
                         w = a.records[slot];
-                        if (PHASE == PHASE_EHS)
-                                eng.walker_ehs(w);
-                        else
-                                eng.walker_del(w);
+                        eng.walker_del(w);
                         if (w.phase == PHASE_DONE) {
                                 walker_store(a, slot, w);
                                 steps_acc += w.t.steps;
@@ -814,6 +828,7 @@ constexpr int MAX_ROUNDS_PER_CHECK = 256;
 /* Launch geometry and optional timing of the stepping kernel. */
 struct Launch {
         int n_cu, block, per_cu, rounds_per_check;
+        unsigned step_grid, del_grid;
         hipStream_t stream;
         hipEvent_t * marks; /* [2 * rounds_per_check] or NULL */
 };
@@ -826,23 +841,32 @@ enum pumas_return launch_round(
         hipLaunchKernelGGL(k_round, dim3(1), dim3(64), 0, stream, a.counters,
             parity);
         if (L.marks) HIP_TRY(hipEventRecord(L.marks[2 * r], stream));
-        hipLaunchKernelGGL(k_step<F32>, dim3((unsigned)(L.n_cu * L.per_cu)),
-            dim3(L.block), 0, stream, a, parity);
+        hipLaunchKernelGGL(k_step<F32>, dim3(L.step_grid), dim3(L.block), 0,
+            stream, a, parity);
         if (L.marks) HIP_TRY(hipEventRecord(L.marks[2 * r + 1], stream));
-        const int n_cu = L.n_cu;
-        hipLaunchKernelGGL((k_event<F32, PHASE_EHS>),
-            dim3((unsigned)(n_cu * 4)), dim3(256), 0, stream, a, parity);
-        hipLaunchKernelGGL((k_event<F32, PHASE_DEL>),
-            dim3((unsigned)(n_cu * 2)), dim3(256), 0, stream, a, parity);
+        hipLaunchKernelGGL(k_del<F32>, dim3(L.del_grid), dim3(256), 0,
+            stream, a, parity);
         HIP_TRY(hipGetLastError());
         return PUMAS_RETURN_SUCCESS;
 }
 
 /* Transport histories [base, base + m) of the batch described by `a0`. */
 enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
-    size_t m, bool f32, const Launch & L, unsigned long long * total_steps,
+    size_t m, bool f32, Launch L, unsigned long long * total_steps,
     unsigned long long * error, struct pumas_gpu_stats * stats)
 {
+        /* No more resident blocks than the chunk fills with histories, so
+         * that a small request takes full wavefronts on few CUs and leaves
+         * the rest of the device to concurrent requests. */
+        const size_t fill = (m + L.block - 1) / L.block;
+        const size_t resident = (size_t)L.n_cu * L.per_cu;
+        L.step_grid = (unsigned)((fill < resident) ? (fill > 0 ? fill : 1) :
+                                                     resident);
+        const size_t del_fill = (m + 255) / 256;
+        const size_t del_max = (size_t)L.n_cu * 2;
+        L.del_grid = (unsigned)((del_fill < del_max) ?
+                (del_fill > 0 ? del_fill : 1) :
+                del_max);
         hipStream_t stream = L.stream;
         KernelArgs a = a0;
         a.states = a0.states + base;
@@ -859,15 +883,14 @@ enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
         const size_t b_records = ((m * sizeof(Walker)) + 255) & ~size_t(255);
         const size_t b_queue = ((m * sizeof(uint32_t)) + 255) & ~size_t(255);
         HIP_TRY(hipMallocAsync(
-            &mem, b_counters + b_records + 4 * b_queue, stream));
+            &mem, b_counters + b_records + 3 * b_queue, stream));
         char * ptr = static_cast<char *>(mem);
         a.counters = reinterpret_cast<unsigned long long *>(ptr);
         a.records = reinterpret_cast<Walker *>(ptr + b_counters);
         char * q = ptr + b_counters + b_records;
         a.q_step[0] = reinterpret_cast<uint32_t *>(q);
         a.q_step[1] = reinterpret_cast<uint32_t *>(q + b_queue);
-        a.q_ehs = reinterpret_cast<uint32_t *>(q + 2 * b_queue);
-        a.q_del = reinterpret_cast<uint32_t *>(q + 3 * b_queue);
+        a.q_del = reinterpret_cast<uint32_t *>(q + 2 * b_queue);
 
         enum pumas_return rc = PUMAS_RETURN_SUCCESS;
         unsigned long long host[C_COUNT];
@@ -1032,8 +1055,8 @@ extern "C" enum pumas_return pumas_gpu_transport(
                 marks.resize(2 * rounds_per_check, nullptr);
                 for (auto & e : marks) HIP_TRY(hipEventCreate(&e));
         }
-        const Launch L = { n_cu, block, per_cu, rounds_per_check, stream,
-                stats ? marks.data() : nullptr };
+        const Launch L = { n_cu, block, per_cu, rounds_per_check, 0, 0,
+                stream, stats ? marks.data() : nullptr };
         for (size_t base = 0; base < n; base += MAX_CHUNK) {
                 const size_t m = (n - base < MAX_CHUNK) ? n - base : MAX_CHUNK;
                 rc = transport_chunk(
