@@ -30,22 +30,37 @@ struct FloatCache {
         }
 };
 
-/* Bracketing row of a value missed by the first cached row. */
+/* Bracketing row of a value missed by the first cached row: second cached
+ * row and neighbours of the first one, then bisection (as
+ * cached_index_miss). */
 PG_FN int findex_miss(const float * table, int n, float value, int * last)
 {
-        if ((value >= table[last[1]]) && (value < table[last[1] + 1]))
-                return last[1];
-        const int imax = n - 1;
-        if (value < table[0]) return -1;
-        if (value >= table[imax]) return imax;
-        int i1 = 0, i2 = imax;
-        do {
-                const int i3 = (i1 + i2) / 2;
-                if (value >= table[i3])
-                        i1 = i3;
-                else
-                        i2 = i3;
-        } while (i2 - i1 >= 2);
+        const int i0 = last[0];
+        const float below = table[(i0 > 0) ? i0 - 1 : 0];
+        const float lo = table[i0];
+        const float hi = table[i0 + 1];
+        const float above = table[(i0 + 2 < n) ? i0 + 2 : n - 1];
+        const float l0 = table[last[1]], l1 = table[last[1] + 1];
+        if ((value >= l0) && (value < l1)) return last[1];
+        int i1;
+        if ((i0 > 0) && (value >= below) && (value < lo)) {
+                i1 = i0 - 1;
+        } else if ((i0 + 2 < n) && (value >= hi) && (value < above)) {
+                i1 = i0 + 1;
+        } else {
+                const int imax = n - 1;
+                if (value < table[0]) return -1;
+                if (value >= table[imax]) return imax;
+                int j1 = 0, j2 = imax;
+                do {
+                        const int j3 = (j1 + j2) / 2;
+                        if (value >= table[j3])
+                                j1 = j3;
+                        else
+                                j2 = j3;
+                } while (j2 - j1 >= 2);
+                i1 = j1;
+        }
         if (i1 != last[0])
                 last[0] = i1;
         else if (i1 != last[1])

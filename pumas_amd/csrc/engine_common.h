@@ -300,19 +300,35 @@ struct Cache {
         }
 };
 
-/* Bracketing row of a value missed by the first cached row. */
+/*
+ * Bracketing row of a value missed by the first cached row. The second
+ * cached row and the two rows next to the first one are checked together
+ * (one round trip to memory): between two lookups of a history the value
+ * usually moves by less than a row. The bisection is the fallback.
+ */
 PG_FN int cached_index_miss(
     const double * table, int n, double value, int * last)
 {
-        if ((value >= table[last[1]]) && (value < table[last[1] + 1]))
-                return last[1];
-        const int i1 = table_index(table, n, value);
-        if ((i1 >= 0) && (i1 < n - 1)) {
-                if (i1 != last[0])
-                        last[0] = i1;
-                else if (i1 != last[1])
-                        last[1] = i1;
+        const int i0 = last[0];
+        const double below = table[(i0 > 0) ? i0 - 1 : 0];
+        const double lo = table[i0];
+        const double hi = table[i0 + 1];
+        const double above = table[(i0 + 2 < n) ? i0 + 2 : n - 1];
+        const double l0 = table[last[1]], l1 = table[last[1] + 1];
+        if ((value >= l0) && (value < l1)) return last[1];
+        int i1;
+        if ((i0 > 0) && (value >= below) && (value < lo)) {
+                i1 = i0 - 1;
+        } else if ((i0 + 2 < n) && (value >= hi) && (value < above)) {
+                i1 = i0 + 1;
+        } else {
+                i1 = table_index(table, n, value);
+                if ((i1 < 0) || (i1 >= n - 1)) return i1;
         }
+        if (i1 != last[0])
+                last[0] = i1;
+        else if (i1 != last[1])
+                last[1] = i1;
         return i1;
 }
 
