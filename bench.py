@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64",
                     help="fp64: reference arithmetic; fp32: FP32-bulk")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay the scheduler rounds as HIP graphs (no "
+                         "per-launch timing of the stepping kernel then)")
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--serve-seconds", type=float, default=0.25,
@@ -197,7 +200,8 @@ def main():
         return pa.transport_device(
             phys, ctx, geom, work.data_ptr(), n, seed=0,
             first_history=k * n, stream=stream, precision=precision,
-            block_size=args.block_size, blocks_per_cu=args.blocks_per_cu)
+            block_size=args.block_size, blocks_per_cu=args.blocks_per_cu,
+            eager=not args.graph)
 
     for k in range(args.warmup):
         st = one_pass(1000 + k)
@@ -224,6 +228,8 @@ def main():
     # each launch, on the stream it runs on).
     step_ms = sum(st.step_kernel_ms for st in passes)
     launches = sum(st.step_launches for st in passes)
+    if step_ms <= 0.0:  # graph replays: no per-launch timing
+        step_ms = sum(st.kernel_ms for st in passes)
     achieved = total_steps * BYTES_PER_STEP_FP64 / (step_ms * 1e-3) / 1e9
 
     traffic = None

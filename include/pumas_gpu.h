@@ -252,8 +252,11 @@ struct pumas_gpu_options {
         /* Safety cap on Monte Carlo steps per history (0 = 10^7). */
         int max_steps;
         /* Scheduler rounds launched between two reads of the queue
-         * lengths (0 = default). */
+         * lengths (0 = default; rounded up to an even number). */
         int rounds_per_check;
+        /* Non zero: launch the rounds one by one instead of replaying them
+         * as a captured HIP graph. */
+        int eager;
 };
 
 /* Optional per-history outputs (device pointers, each may be NULL). */
@@ -270,7 +273,8 @@ struct pumas_gpu_stats {
         uint64_t steps;         /* total Monte Carlo steps */
         uint64_t histories;     /* histories transported */
         double kernel_ms;       /* transport kernel time (HIP events) */
-        double step_kernel_ms;  /* time in the stepping kernel */
+        double step_kernel_ms;  /* time in the stepping kernel (eager
+                                   launches only; 0 with graph replays) */
         uint64_t step_launches; /* launches of the stepping kernel */
         uint64_t rounds;        /* host checks of the queue lengths */
 };

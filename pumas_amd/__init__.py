@@ -290,19 +290,21 @@ class TransportStats:
 
 
 def _options(precision, block_size, blocks_per_cu, max_steps,
-             rounds_per_check=0):
+             rounds_per_check=0, eager=False):
     o = _abi.GpuOptions()
     o.precision = int(precision)
     o.block_size = block_size
     o.blocks_per_cu = blocks_per_cu
     o.max_steps = max_steps
     o.rounds_per_check = rounds_per_check
+    o.eager = int(bool(eager))
     return o
 
 
 def transport(physics, context, geometry, states, seed=0, first_history=0,
               outputs=True, precision=Precision.FP64, block_size=0,
-              blocks_per_cu=0, max_steps=0, raise_errors=True):
+              blocks_per_cu=0, max_steps=0, raise_errors=True,
+              rounds_per_check=0, eager=False):
     """Transport host `states` (a STATE_DTYPE array, updated in place).
 
     Batch counterpart of pumas_context_transport: history i draws from the
@@ -329,7 +331,8 @@ def transport(physics, context, geometry, states, seed=0, first_history=0,
         out.draws = res.draws.ctypes.data
         out.rc = res.rc.ctypes.data
     stats = _abi.GpuStats()
-    opts = _options(precision, block_size, blocks_per_cu, max_steps)
+    opts = _options(precision, block_size, blocks_per_cu, max_steps,
+                    rounds_per_check, eager)
     ctx = context.to_c()
     rc = lib.pumas_gpu_transport_host(
         physics.handle, ctypes.byref(ctx), geometry.handle,
@@ -346,7 +349,7 @@ def transport(physics, context, geometry, states, seed=0, first_history=0,
 def transport_device(physics, context, geometry, states_ptr, n, seed=0,
                      first_history=0, outputs=None, precision=Precision.FP64,
                      block_size=0, blocks_per_cu=0, max_steps=0, stream=None,
-                     stats=True, rounds_per_check=0):
+                     stats=True, rounds_per_check=0, eager=False):
     """Transport `n` states already in device memory at `states_ptr`.
 
     `outputs` maps any of "events", "media", "steps", "draws", "rc" to a
@@ -359,7 +362,7 @@ def transport_device(physics, context, geometry, states_ptr, n, seed=0,
         setattr(out, k, v)
     st = _abi.GpuStats()
     opts = _options(precision, block_size, blocks_per_cu, max_steps,
-                    rounds_per_check)
+                    rounds_per_check, eager)
     ctx = context.to_c()
     _check(lib.pumas_gpu_transport(
         physics.handle, ctypes.byref(ctx), geometry.handle,

@@ -828,6 +828,7 @@ constexpr int MAX_ROUNDS_PER_CHECK = 256;
 /* Launch geometry and optional timing of the stepping kernel. */
 struct Launch {
         int n_cu, block, per_cu, rounds_per_check;
+        bool graph;
         unsigned step_grid, del_grid;
         hipStream_t stream;
         hipEvent_t * marks; /* [2 * rounds_per_check] or NULL */
@@ -908,6 +909,34 @@ enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
                 return pg_fail(PUMAS_RETURN_VALUE_ERROR,
                     "transport kernel launch failed");
         }
+        /* The sequence of rounds between two checks is captured once in a
+         * HIP graph and replayed (an even number of rounds, so that every
+         * replay starts on step queue 0): one launch per check instead of
+         * three per round. Counters are cleared by kernels, not memset
+         * nodes. */
+        hipGraphExec_t exec = nullptr;
+        if (L.graph) {
+                hipGraph_t graph = nullptr;
+                HIP_TRY(hipStreamBeginCapture(
+                    stream, hipStreamCaptureModeThreadLocal));
+                int p = 0;
+                for (int r = 0; r < L.rounds_per_check; r++, p ^= 1) {
+                        rc = f32 ? launch_round<true>(a, p, L, r) :
+                                   launch_round<false>(a, p, L, r);
+                        if (rc != PUMAS_RETURN_SUCCESS) break;
+                }
+                const hipError_t e = hipStreamEndCapture(stream, &graph);
+                if (rc != PUMAS_RETURN_SUCCESS) {
+                        if (graph) (void)hipGraphDestroy(graph);
+                        (void)hipFreeAsync(mem, stream);
+                        return rc;
+                }
+                HIP_TRY(e);
+                const hipError_t ei =
+                    hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                HIP_TRY(ei);
+        }
         int parity = 0, launched = 0;
         for (;;) {
                 HIP_TRY(hipMemcpyAsync(host, a.counters, sizeof(host),
@@ -925,6 +954,11 @@ enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
                 launched = 0;
                 if (host[C_STEP_N0 + parity] == 0) break;
                 if (stats) stats->rounds += 1;
+                if (exec) {
+                        HIP_TRY(hipGraphLaunch(exec, stream));
+                        launched = L.rounds_per_check;
+                        continue;
+                }
                 for (int r = 0; r < L.rounds_per_check; r++) {
                         rc = f32 ? launch_round<true>(a, parity, L, r) :
                                    launch_round<false>(a, parity, L, r);
@@ -934,6 +968,7 @@ enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
                 }
                 if (rc != PUMAS_RETURN_SUCCESS) break;
         }
+        if (exec) (void)hipGraphExecDestroy(exec);
         *total_steps += host[C_STEPS];
         if ((*error == 0) && (host[C_ERROR] != 0)) *error = host[C_ERROR];
         HIP_TRY(hipFreeAsync(mem, stream));
@@ -1032,10 +1067,10 @@ extern "C" enum pumas_return pumas_gpu_transport(
         const int per_cu = (options && options->blocks_per_cu > 0) ?
             options->blocks_per_cu :
             2;
-        const int rounds_per_check =
-            (options && options->rounds_per_check > 0) ?
+        int rounds_per_check = (options && options->rounds_per_check > 0) ?
             options->rounds_per_check :
             ROUNDS_PER_CHECK;
+        rounds_per_check += rounds_per_check & 1; /* even, for the graph */
         if (rounds_per_check > MAX_ROUNDS_PER_CHECK)
                 return pg_fail(PUMAS_RETURN_VALUE_ERROR,
                     "rounds_per_check must be at most %d",
@@ -1049,14 +1084,20 @@ extern "C" enum pumas_return pumas_gpu_transport(
         }
         const bool f32 = (precision == PUMAS_GPU_PRECISION_FP32_BULK);
         unsigned long long total_steps = 0, error = 0;
+        /* The legacy NULL stream cannot be captured. */
+        const bool graph = !(options && options->eager) && (stream != nullptr);
         std::vector<hipEvent_t> marks;
         if (stats) {
                 memset(stats, 0x0, sizeof(*stats));
-                marks.resize(2 * rounds_per_check, nullptr);
-                for (auto & e : marks) HIP_TRY(hipEventCreate(&e));
+                if (!graph) {
+                        marks.resize(2 * rounds_per_check, nullptr);
+                        for (auto & e : marks) HIP_TRY(hipEventCreate(&e));
+                }
         }
-        const Launch L = { n_cu, block, per_cu, rounds_per_check, 0, 0,
-                stream, stats ? marks.data() : nullptr };
+        /* Per-launch timing of the stepping kernel needs eager launches:
+         * events recorded inside a graph cannot be timed. */
+        const Launch L = { n_cu, block, per_cu, rounds_per_check, graph, 0, 0,
+                stream, (stats && !graph) ? marks.data() : nullptr };
         for (size_t base = 0; base < n; base += MAX_CHUNK) {
                 const size_t m = (n - base < MAX_CHUNK) ? n - base : MAX_CHUNK;
                 rc = transport_chunk(
@@ -1108,9 +1149,15 @@ extern "C" enum pumas_return pumas_gpu_transport_host(
                 if (outputs->rc)
                         HIP_TRY(hipMalloc(&d_out.rc, ns * sizeof(int32_t)));
         }
+        /* A stream of its own: the legacy NULL stream cannot be captured
+         * in a graph. */
+        hipStream_t stream = nullptr;
+        HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         const enum pumas_return rc = pumas_gpu_transport(physics, context,
             geometry, d_states, n, seed, first_history, &d_out, options, stats,
-            nullptr);
+            stream);
+        (void)hipStreamSynchronize(stream);
+        (void)hipStreamDestroy(stream);
         const std::string message = g_error;
         HIP_TRY(hipMemcpy(states, d_states, n * sizeof(*states),
             hipMemcpyDeviceToHost));

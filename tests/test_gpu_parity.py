@@ -186,6 +186,28 @@ def test_batch_split_invariance(gpu_standard):
 
 
 @pytest.mark.parametrize("energy_loss", [pa.Mode.STRAGGLED, pa.Mode.MIXED])
+def test_graph_replays_equal_eager_launches(gpu_standard, energy_loss):
+    """The rounds between two host checks are replayed as a captured HIP
+    graph. With two rounds per check a batch takes many replays, each
+    separated by the host's counter copy and synchronisation; every call
+    must equal the same transport launched round by round, and repeated
+    calls (fresh graphs) must equal each other."""
+    s = W.slabs_backward_states(3000, seed=12)
+    geom = pa.Geometry(gpu_standard, W.slabs_geometry())
+    ctx = W.slabs_backward_context(energy_loss=energy_loss)
+    eager = pa.transport(gpu_standard, ctx, geom, s.copy(), seed=12,
+                         rounds_per_check=2, eager=True)
+    for _ in range(3):
+        g = pa.transport(gpu_standard, ctx, geom, s.copy(), seed=12,
+                         rounds_per_check=2)
+        assert (g.states == eager.states).all()
+        for k in ("events", "media", "steps", "draws", "rc"):
+            np.testing.assert_array_equal(getattr(g, k), getattr(eager, k))
+    g = pa.transport(gpu_standard, ctx, geom, s.copy(), seed=12)
+    assert (g.states == eager.states).all()
+
+
+@pytest.mark.parametrize("energy_loss", [pa.Mode.STRAGGLED, pa.Mode.MIXED])
 def test_backward_weights_vs_aligned_reference(ref_standard_aligned,
                                                gpu_standard, energy_loss):
     """Backward DEL weights at the FP64 tolerance, against the reference
