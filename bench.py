@@ -53,6 +53,10 @@ def parse():
     ap.add_argument("--graph", action="store_true",
                     help="replay the scheduler rounds as HIP graphs (no "
                          "per-launch timing of the stepping kernel then)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (default launch settings)")
+    ap.add_argument("--tune-n", type=int, default=2_000_000,
+                    help="histories of the autotuner's sample")
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--serve-seconds", type=float, default=0.25,
@@ -195,13 +199,23 @@ def main():
     precision = pa.Precision.FP64 if args.precision == "fp64" else \
         pa.Precision.FP32_BULK
 
+    tuned = {"block_size": args.block_size,
+             "blocks_per_cu": args.blocks_per_cu, "ehs_batch": 0}
+    if not args.no_tune and args.block_size == 0 and args.blocks_per_cu == 0:
+        m = min(n, args.tune_n)
+        tuned, rate = pa.autotune(phys, ctx, geom, pristine.data_ptr(), m,
+                                  seed=7, precision=precision, stream=stream)
+        log(f"bench: autotuned on {m} histories: {tuned} "
+            f"({rate:.3e} steps/s on the sample)")
+
     def one_pass(k):
         work.copy_(pristine)
         return pa.transport_device(
             phys, ctx, geom, work.data_ptr(), n, seed=0,
             first_history=k * n, stream=stream, precision=precision,
-            block_size=args.block_size, blocks_per_cu=args.blocks_per_cu,
-            eager=not args.graph)
+            block_size=tuned["block_size"],
+            blocks_per_cu=tuned["blocks_per_cu"],
+            ehs_batch=tuned["ehs_batch"], eager=not args.graph)
 
     for k in range(args.warmup):
         st = one_pass(1000 + k)
@@ -270,6 +284,7 @@ def main():
                         "[0.5, 1]",
             "histories": n,
             "precision": args.precision,
+            "launch": tuned,
         },
         "roofline": {
             "bound": "hbm",

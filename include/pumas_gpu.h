@@ -257,7 +257,25 @@ struct pumas_gpu_options {
         /* Non zero: launch the rounds one by one instead of replaying them
          * as a captured HIP graph. */
         int eager;
+        /* Lanes of a stepping wavefront that wait for an elastic hard
+         * scattering before the wavefront runs them (0 = 16, at most 64). */
+        int ehs_batch;
 };
+
+/*
+ * Autotuner: times pumas_gpu_transport on a sample of `n` device-resident
+ * states (left unchanged) for each launch shape (block size, resident
+ * blocks per CU) and EHS batch size of a fixed candidate list, and writes
+ * the fastest into `best` (its precision is `precision`), with its rate in
+ * Monte Carlo steps per second. The sample should hold at least ~10^6
+ * histories of the workload to tune (several times the 131072 resident
+ * lanes), so that the tail of a launch does not dominate the timing.
+ */
+enum pumas_return pumas_gpu_autotune(const pumas_gpu_physics * physics,
+    const struct pumas_gpu_context * context,
+    const pumas_gpu_geometry * geometry, const struct pumas_state * states,
+    size_t n, uint64_t seed, enum pumas_gpu_precision precision,
+    struct pumas_gpu_options * best, double * steps_per_s, void * stream);
 
 /* Optional per-history outputs (device pointers, each may be NULL). */
 struct pumas_gpu_outputs {

@@ -26,7 +26,7 @@ __all__ = [
     "GeometryKind", "Mode", "Physics", "Precision", "PumasError", "Return",
     "STATE_DTYPE", "Server", "Step", "Tally", "TallySpec", "library",
     "new_states",
-    "physics_path", "tally", "tally_device", "transport",
+    "autotune", "physics_path", "tally", "tally_device", "transport",
     "transport_device",
 ]
 
@@ -290,7 +290,7 @@ class TransportStats:
 
 
 def _options(precision, block_size, blocks_per_cu, max_steps,
-             rounds_per_check=0, eager=False):
+             rounds_per_check=0, eager=False, ehs_batch=0):
     o = _abi.GpuOptions()
     o.precision = int(precision)
     o.block_size = block_size
@@ -298,13 +298,14 @@ def _options(precision, block_size, blocks_per_cu, max_steps,
     o.max_steps = max_steps
     o.rounds_per_check = rounds_per_check
     o.eager = int(bool(eager))
+    o.ehs_batch = ehs_batch
     return o
 
 
 def transport(physics, context, geometry, states, seed=0, first_history=0,
               outputs=True, precision=Precision.FP64, block_size=0,
               blocks_per_cu=0, max_steps=0, raise_errors=True,
-              rounds_per_check=0, eager=False):
+              rounds_per_check=0, eager=False, ehs_batch=0):
     """Transport host `states` (a STATE_DTYPE array, updated in place).
 
     Batch counterpart of pumas_context_transport: history i draws from the
@@ -332,7 +333,7 @@ def transport(physics, context, geometry, states, seed=0, first_history=0,
         out.rc = res.rc.ctypes.data
     stats = _abi.GpuStats()
     opts = _options(precision, block_size, blocks_per_cu, max_steps,
-                    rounds_per_check, eager)
+                    rounds_per_check, eager, ehs_batch)
     ctx = context.to_c()
     rc = lib.pumas_gpu_transport_host(
         physics.handle, ctypes.byref(ctx), geometry.handle,
@@ -349,7 +350,8 @@ def transport(physics, context, geometry, states, seed=0, first_history=0,
 def transport_device(physics, context, geometry, states_ptr, n, seed=0,
                      first_history=0, outputs=None, precision=Precision.FP64,
                      block_size=0, blocks_per_cu=0, max_steps=0, stream=None,
-                     stats=True, rounds_per_check=0, eager=False):
+                     stats=True, rounds_per_check=0, eager=False,
+                     ehs_batch=0):
     """Transport `n` states already in device memory at `states_ptr`.
 
     `outputs` maps any of "events", "media", "steps", "draws", "rc" to a
@@ -362,7 +364,7 @@ def transport_device(physics, context, geometry, states_ptr, n, seed=0,
         setattr(out, k, v)
     st = _abi.GpuStats()
     opts = _options(precision, block_size, blocks_per_cu, max_steps,
-                    rounds_per_check, eager)
+                    rounds_per_check, eager, ehs_batch)
     ctx = context.to_c()
     _check(lib.pumas_gpu_transport(
         physics.handle, ctypes.byref(ctx), geometry.handle,
@@ -565,3 +567,22 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def autotune(physics, context, geometry, states_ptr, n, seed=0,
+             precision=Precision.FP64, stream=None):
+    """Launch settings (block_size, blocks_per_cu, ehs_batch) of the fastest
+    candidate on a sample of `n` device-resident states (left unchanged),
+    and its rate in Monte Carlo steps per second."""
+    lib = library()
+    best = _abi.GpuOptions()
+    rate = ctypes.c_double()
+    ctx = context.to_c()
+    _check(lib.pumas_gpu_autotune(
+        physics.handle, ctypes.byref(ctx), geometry.handle,
+        ctypes.c_void_p(states_ptr), n, seed, int(precision),
+        ctypes.byref(best), ctypes.byref(rate),
+        ctypes.c_void_p(stream) if stream else None))
+    return ({"block_size": best.block_size,
+             "blocks_per_cu": best.blocks_per_cu,
+             "ehs_batch": best.ehs_batch}, float(rate.value))

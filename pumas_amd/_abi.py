@@ -146,7 +146,8 @@ class GpuOutputs(ctypes.Structure):
 class GpuOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("block_size", ctypes.c_int),
                 ("blocks_per_cu", ctypes.c_int), ("max_steps", ctypes.c_int),
-                ("rounds_per_check", ctypes.c_int), ("eager", ctypes.c_int)]
+                ("rounds_per_check", ctypes.c_int), ("eager", ctypes.c_int),
+                ("ehs_batch", ctypes.c_int)]
 
 
 class GpuStats(ctypes.Structure):
@@ -223,6 +224,7 @@ EXPORTED_SYMBOLS = (
     "pumas_gpu_transport_host",
     "pumas_gpu_tally",
     "pumas_gpu_tally_host",
+    "pumas_gpu_autotune",
     "pumas_gpu_server_create",
     "pumas_gpu_server_destroy",
     "pumas_gpu_server_submit",
@@ -269,6 +271,10 @@ def declare(lib):
     lib.pumas_gpu_tally_host.argtypes = [vp, vp, ctypes.c_size_t,
                                          P(TallySpec), P(TallyResult)]
     lib.pumas_gpu_tally_host.restype = ctypes.c_int
+    lib.pumas_gpu_autotune.argtypes = [
+        vp, P(GpuContext), vp, vp, ctypes.c_size_t, ctypes.c_uint64,
+        ctypes.c_int, P(GpuOptions), P(ctypes.c_double), vp]
+    lib.pumas_gpu_autotune.restype = ctypes.c_int
     lib.pumas_gpu_server_create.argtypes = [P(ServerOptions), P(vp)]
     lib.pumas_gpu_server_create.restype = ctypes.c_int
     lib.pumas_gpu_server_destroy.argtypes = [P(vp)]

@@ -550,6 +550,7 @@ struct KernelArgs {
         uint32_t * draws;
         int32_t * rc;
         int max_steps;
+        int ehs_batch;
         unsigned long long * counters;
         Walker * records;     /* [n] histories in flight */
         uint32_t * q_step[2]; /* [n] step queues (ping-pong) */
@@ -676,8 +677,8 @@ __global__ void __launch_bounds__(256) k_init(KernelArgs args)
             w.phase == PHASE_STEP, (uint32_t)i);
 }
 
-/* Maximum number of lanes of a wavefront waiting for an elastic hard
- * scattering before the wavefront processes them together. */
+/* Default maximum number of lanes of a wavefront waiting for an elastic
+ * hard scattering before the wavefront processes them together. */
 constexpr int EHS_BATCH = 16;
 
 /* Clears the per-round counters before round `parity`. */
@@ -750,15 +751,15 @@ __global__ void __launch_bounds__(256, 2) k_step(KernelArgs args, int parity)
 
                 /* Elastic hard scatterings, on ~16% of the steps of a muon
                  * in rock, run once enough lanes wait for one (half of the
-                 * lanes in flight, at most EHS_BATCH) or none can step, so
+                 * lanes in flight, at most a.ehs_batch) or none can step, so
                  * that the wavefront runs that code with most lanes busy. */
                 const uint64_t ehs = __ballot(w.phase == PHASE_EHS);
                 const uint64_t stepping = __ballot(w.phase == PHASE_STEP);
                 const int waiting = __popcll(ehs);
                 const int in_flight = waiting + __popcll(stepping);
-                const int batch = (in_flight + 1) / 2 < EHS_BATCH ?
+                const int batch = (in_flight + 1) / 2 < a.ehs_batch ?
                     (in_flight + 1) / 2 :
-                    EHS_BATCH;
+                    a.ehs_batch;
                 if (ehs && ((waiting >= batch) || !stepping)) {
                         if (w.phase == PHASE_EHS) eng.walker_ehs(w);
                 }
@@ -1052,6 +1053,11 @@ extern "C" enum pumas_return pumas_gpu_transport(
         }
         a.max_steps = (options && options->max_steps > 0) ? options->max_steps :
                                                             10000000;
+        a.ehs_batch = (options && options->ehs_batch > 0) ? options->ehs_batch :
+                                                            EHS_BATCH;
+        if (a.ehs_batch > 64)
+                return pg_fail(PUMAS_RETURN_VALUE_ERROR,
+                    "ehs_batch must be at most 64 (got %d)", a.ehs_batch);
 
         int device;
         HIP_TRY(hipGetDevice(&device));

@@ -4,6 +4,13 @@ import sys
 import numpy as np
 import pytest
 
+# torch brings its own HIP runtime; it must be loaded before libpumas_hip in
+# a process that uses both (device buffers of the GPU tests).
+try:
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover
+    torch = None
+
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, REPO)
 sys.path.insert(0, os.path.join(REPO, "oracle"))

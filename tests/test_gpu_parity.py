@@ -222,3 +222,39 @@ def test_backward_weights_vs_aligned_reference(ref_standard_aligned,
     gpu = pa.transport(gpu_standard, ctx, pa.Geometry(gpu_standard, desc),
                        s.copy(), seed=11)
     assert_parity(gpu, ref, RTOL_STEPPING)
+
+
+@pytest.mark.parametrize("block_size,blocks_per_cu,ehs_batch",
+                         [(64, 8, 8), (128, 4, 32), (256, 1, 1)])
+def test_launch_settings_invariance(gpu_standard, block_size, blocks_per_cu,
+                                    ehs_batch):
+    """Launch shape and EHS batching change the interleaving of histories,
+    never their results."""
+    s = W.slabs_forward_states(4000, seed=13)
+    geom = pa.Geometry(gpu_standard, W.slabs_geometry())
+    ctx = W.slabs_forward_context()
+    ref = pa.transport(gpu_standard, ctx, geom, s.copy(), seed=13)
+    alt = pa.transport(gpu_standard, ctx, geom, s.copy(), seed=13,
+                       block_size=block_size, blocks_per_cu=blocks_per_cu,
+                       ehs_batch=ehs_batch)
+    assert (alt.states == ref.states).all()
+    for k in ("events", "media", "steps", "draws", "rc"):
+        np.testing.assert_array_equal(getattr(alt, k), getattr(ref, k))
+
+
+def test_autotune(gpu_standard):
+    import torch
+    s = W.slabs_forward_states(50_000, seed=14)
+    geom = pa.Geometry(gpu_standard, W.slabs_geometry())
+    ctx = W.slabs_forward_context()
+    d = torch.from_numpy(s.view(np.uint8).reshape(-1)).cuda()
+    before = d.clone()
+    best, rate = pa.autotune(gpu_standard, ctx, geom, d.data_ptr(), len(s))
+    torch.cuda.synchronize()
+    assert torch.equal(d, before)
+    assert best["block_size"] in (64, 128, 256)
+    assert best["ehs_batch"] in (8, 16, 24, 32)
+    assert rate > 0.0
+    tuned = pa.transport(gpu_standard, ctx, geom, s.copy(), **best)
+    ref = pa.transport(gpu_standard, ctx, geom, s.copy())
+    assert (tuned.states == ref.states).all()
