@@ -155,6 +155,7 @@ void pumas_gpu_context_default(struct pumas_gpu_context * context);
  * Physics tables.
  * ------------------------------------------------------------------------ */
 typedef struct pumas_gpu_physics pumas_gpu_physics;
+typedef struct pumas_gpu_recorder pumas_gpu_recorder;
 
 /*
  * Upload the physics tables of a PTAB file, once. The file is written by
@@ -260,6 +261,8 @@ struct pumas_gpu_options {
         /* Lanes of a stepping wavefront that wait for an elastic hard
          * scattering before the wavefront runs them (0 = 16, at most 64). */
         int ehs_batch;
+        /* Frame recorder, or NULL (see below). */
+        const struct pumas_gpu_recorder * recorder;
 };
 
 /*
@@ -331,6 +334,36 @@ enum pumas_return pumas_gpu_transport_host(const pumas_gpu_physics * physics,
     size_t n, uint64_t seed, uint64_t first_history,
     const struct pumas_gpu_outputs * outputs,
     const struct pumas_gpu_options * options, struct pumas_gpu_stats * stats);
+
+/* ------------------------------------------------------------------------
+ * Frame recorder, the device counterpart of the reference's pumas_recorder
+ * (include/pumas.h:378-458; src/pumas.c:2797-2851 create/clear/destroy,
+ * 7738-7780 recording). Histories with ids in [first_history, first_history
+ * + n_histories) record frames at the reference's points (track start and
+ * stop, every `period`-th step, DEL pre and post points, medium changes;
+ * only events and medium changes when `period` <= 0) into a device buffer,
+ * in the order recorded; frames beyond the capacity are counted and
+ * dropped. Pass the recorder in pumas_gpu_options.recorder. Replaces:
+ * `context->recorder` (include/pumas.h:668).
+ * ------------------------------------------------------------------------ */
+struct pumas_gpu_frame {
+        struct pumas_state state; /* 104 bytes */
+        int32_t medium;   /* medium index, -1 = outside (a NULL medium) */
+        int32_t event;    /* enum pumas_event of the frame */
+        uint64_t history; /* history id */
+        uint64_t reserved;
+};
+
+enum pumas_return pumas_gpu_recorder_create(size_t capacity, int period,
+    uint64_t first_history, uint64_t n_histories,
+    pumas_gpu_recorder ** recorder);
+void pumas_gpu_recorder_destroy(pumas_gpu_recorder ** recorder);
+/* Forget the recorded frames (replaces pumas_recorder_clear). */
+enum pumas_return pumas_gpu_recorder_clear(pumas_gpu_recorder * recorder);
+/* Copy up to `max_frames` frames to host memory; `length` receives the
+ * number of frames recorded (above the capacity if some were dropped). */
+enum pumas_return pumas_gpu_recorder_read(const pumas_gpu_recorder * recorder,
+    struct pumas_gpu_frame * frames, size_t max_frames, uint64_t * length);
 
 /* ------------------------------------------------------------------------
  * Tallies: flux estimators of transported histories, on the device.

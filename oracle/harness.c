@@ -268,7 +268,21 @@ struct ref_job {
         int thread;
         int n_threads;
         uint64_t total_steps;
+        /* Recording (single thread): frames of every history, as the
+         * reference's default in-memory recorder (src/pumas.c:7738-7780). */
+        int rec_period;
+        int64_t rec_capacity, rec_count;
+        struct pumas_state * rec_states;
+        int * rec_media;
+        int * rec_events;
+        uint64_t * rec_history;
 };
+
+static int ref_medium_index(
+    const struct ref_geometry * g, const struct pumas_medium * m)
+{
+        return (m == NULL) ? -1 : (int)((const struct ref_medium *)m - g->media);
+}
 
 static void * ref_worker(void * arg)
 {
@@ -293,6 +307,16 @@ static void * ref_worker(void * arg)
         struct ref_user * u = context->user_data;
         u->geometry = job->geometry;
         u->seed = job->seed;
+        struct pumas_recorder * recorder = NULL;
+        if (job->rec_states != NULL) {
+                if (pumas_recorder_create(&recorder, 0) !=
+                    PUMAS_RETURN_SUCCESS) {
+                        pumas_context_destroy(&context);
+                        return NULL;
+                }
+                recorder->period = job->rec_period;
+                context->recorder = recorder;
+        }
 
         int64_t i;
         for (i = job->thread; i < job->n; i += job->n_threads) {
@@ -331,7 +355,21 @@ static void * ref_worker(void * arg)
                 if (job->draws) job->draws[i] = u->draws;
                 if (job->rc) job->rc[i] = rc;
                 job->total_steps += u->steps;
+                if (recorder != NULL) {
+                        const struct pumas_frame * f;
+                        for (f = recorder->first; f != NULL; f = f->next) {
+                                const int64_t k = job->rec_count++;
+                                if (k >= job->rec_capacity) continue;
+                                job->rec_states[k] = f->state;
+                                job->rec_media[k] =
+                                    ref_medium_index(job->geometry, f->medium);
+                                job->rec_events[k] = f->event;
+                                job->rec_history[k] = u->history;
+                        }
+                        pumas_recorder_clear(recorder);
+                }
         }
+        if (recorder != NULL) pumas_recorder_destroy(&recorder);
         pumas_context_destroy(&context);
         return NULL;
 }
@@ -447,3 +485,41 @@ double ref_dcs_exact(const void * physics, int ip, int iel, double K, double q)
 
 /* math_dilog, src/pumas.c:12609. */
 double ref_dilog(double x) { return math_dilog(x); }
+
+/*
+ * Transport `n` histories on one thread with the reference's default
+ * recorder at `period`, writing up to `capacity` frames (state, medium index,
+ * event, history id) in the order recorded. Returns the number of frames
+ * recorded (more than `capacity` if some were dropped).
+ */
+int64_t ref_transport_record(const void * physics,
+    const struct pumas_gpu_context * config, const void * geometry,
+    struct pumas_state * states, int64_t n, uint64_t seed, uint64_t first,
+    int period, int64_t capacity, struct pumas_state * frame_states,
+    int * frame_media, int * frame_events, uint64_t * frame_history)
+{
+        struct sigaction sa, previous;
+        memset(&sa, 0x0, sizeof(sa));
+        sa.sa_handler = &ref_abort_handler;
+        sigemptyset(&sa.sa_mask);
+        sigaction(SIGABRT, &sa, &previous);
+        struct ref_job job;
+        memset(&job, 0x0, sizeof(job));
+        job.physics = physics;
+        job.config = config;
+        job.geometry = geometry;
+        job.states = states;
+        job.n = n;
+        job.seed = seed;
+        job.first = first;
+        job.n_threads = 1;
+        job.rec_period = period;
+        job.rec_capacity = capacity;
+        job.rec_states = frame_states;
+        job.rec_media = frame_media;
+        job.rec_events = frame_events;
+        job.rec_history = frame_history;
+        ref_worker(&job);
+        sigaction(SIGABRT, &previous, NULL);
+        return job.rec_count;
+}

@@ -202,3 +202,32 @@ def flux(kind, cos_theta, kinetic, charge):
                                       out.ctypes.data)
     assert rc == 0
     return out
+
+
+FRAME_FIELDS = ("states", "media", "events", "history")
+
+
+def transport_record(physics, context, geometry, states, seed=0,
+                     first_history=0, period=1, capacity=1 << 20):
+    """Transport on one thread with the reference's recorder (period as
+    `pumas_recorder.period`); returns (final states, frames) with frames a
+    dict of arrays: states, media (index, -1 = outside), events, history."""
+    lib = library(physics.aligned)
+    f = lib.ref_transport_record
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                  ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64,
+                  ctypes.c_uint64, ctypes.c_int, ctypes.c_int64] + \
+        [ctypes.c_void_p] * 4
+    f.restype = ctypes.c_int64
+    s = np.ascontiguousarray(states, dtype=STATE_DTYPE).copy()
+    fs = np.zeros(capacity, STATE_DTYPE)
+    fm = np.zeros(capacity, np.int32)
+    fe = np.zeros(capacity, np.int32)
+    fh = np.zeros(capacity, np.uint64)
+    ctx = context.to_c()
+    count = f(physics.handle, ctypes.byref(ctx), geometry.handle,
+              s.ctypes.data, len(s), seed, first_history, period, capacity,
+              fs.ctypes.data, fm.ctypes.data, fe.ctypes.data, fh.ctypes.data)
+    k = min(count, capacity)
+    return s, {"states": fs[:k], "media": fm[:k], "events": fe[:k],
+               "history": fh[:k], "count": count}

@@ -23,6 +23,7 @@ from ._abi import (Event, Flux, GeometryKind, Mode, Precision, Return, Step,
 
 __all__ = [
     "Context", "Event", "Flux", "Geometry", "GeometryDescription",
+    "Recorder",
     "GeometryKind", "Mode", "Physics", "Precision", "PumasError", "Return",
     "STATE_DTYPE", "Server", "Step", "Tally", "TallySpec", "library",
     "new_states",
@@ -290,7 +291,7 @@ class TransportStats:
 
 
 def _options(precision, block_size, blocks_per_cu, max_steps,
-             rounds_per_check=0, eager=False, ehs_batch=0):
+             rounds_per_check=0, eager=False, ehs_batch=0, recorder=None):
     o = _abi.GpuOptions()
     o.precision = int(precision)
     o.block_size = block_size
@@ -299,13 +300,14 @@ def _options(precision, block_size, blocks_per_cu, max_steps,
     o.rounds_per_check = rounds_per_check
     o.eager = int(bool(eager))
     o.ehs_batch = ehs_batch
+    o.recorder = recorder.handle if recorder is not None else None
     return o
 
 
 def transport(physics, context, geometry, states, seed=0, first_history=0,
               outputs=True, precision=Precision.FP64, block_size=0,
               blocks_per_cu=0, max_steps=0, raise_errors=True,
-              rounds_per_check=0, eager=False, ehs_batch=0):
+              rounds_per_check=0, eager=False, ehs_batch=0, recorder=None):
     """Transport host `states` (a STATE_DTYPE array, updated in place).
 
     Batch counterpart of pumas_context_transport: history i draws from the
@@ -333,7 +335,7 @@ def transport(physics, context, geometry, states, seed=0, first_history=0,
         out.rc = res.rc.ctypes.data
     stats = _abi.GpuStats()
     opts = _options(precision, block_size, blocks_per_cu, max_steps,
-                    rounds_per_check, eager, ehs_batch)
+                    rounds_per_check, eager, ehs_batch, recorder)
     ctx = context.to_c()
     rc = lib.pumas_gpu_transport_host(
         physics.handle, ctypes.byref(ctx), geometry.handle,
@@ -351,7 +353,7 @@ def transport_device(physics, context, geometry, states_ptr, n, seed=0,
                      first_history=0, outputs=None, precision=Precision.FP64,
                      block_size=0, blocks_per_cu=0, max_steps=0, stream=None,
                      stats=True, rounds_per_check=0, eager=False,
-                     ehs_batch=0):
+                     ehs_batch=0, recorder=None):
     """Transport `n` states already in device memory at `states_ptr`.
 
     `outputs` maps any of "events", "media", "steps", "draws", "rc" to a
@@ -364,7 +366,7 @@ def transport_device(physics, context, geometry, states_ptr, n, seed=0,
         setattr(out, k, v)
     st = _abi.GpuStats()
     opts = _options(precision, block_size, blocks_per_cu, max_steps,
-                    rounds_per_check, eager, ehs_batch)
+                    rounds_per_check, eager, ehs_batch, recorder)
     ctx = context.to_c()
     _check(lib.pumas_gpu_transport(
         physics.handle, ctypes.byref(ctx), geometry.handle,
@@ -586,3 +588,43 @@ def autotune(physics, context, geometry, states_ptr, n, seed=0,
     return ({"block_size": best.block_size,
              "blocks_per_cu": best.blocks_per_cu,
              "ehs_batch": best.ehs_batch}, float(rate.value))
+
+
+class Recorder:
+    """Device frame recorder (pumas_gpu_recorder), the counterpart of the
+    reference's pumas_recorder: frames of histories with ids in
+    [first_history, first_history + n_histories), every `period`-th step
+    plus track start/stop, DEL and medium-change points."""
+
+    def __init__(self, capacity, period=1, first_history=0, n_histories=1):
+        lib = library()
+        h = ctypes.c_void_p()
+        _check(lib.pumas_gpu_recorder_create(capacity, period, first_history,
+                                             n_histories, ctypes.byref(h)))
+        self.handle = h
+        self.capacity = capacity
+
+    def frames(self):
+        """Recorded frames (FRAME_DTYPE array, in recording order) and the
+        number of frames recorded, dropped ones included."""
+        lib = library()
+        out = np.zeros(self.capacity, _abi.FRAME_DTYPE)
+        length = ctypes.c_uint64()
+        _check(lib.pumas_gpu_recorder_read(self.handle, out.ctypes.data,
+                                           self.capacity,
+                                           ctypes.byref(length)))
+        return out[:min(length.value, self.capacity)], int(length.value)
+
+    def clear(self):
+        _check(library().pumas_gpu_recorder_clear(self.handle))
+
+    def close(self):
+        if self.handle:
+            library().pumas_gpu_recorder_destroy(ctypes.byref(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

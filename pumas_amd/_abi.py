@@ -147,7 +147,7 @@ class GpuOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("block_size", ctypes.c_int),
                 ("blocks_per_cu", ctypes.c_int), ("max_steps", ctypes.c_int),
                 ("rounds_per_check", ctypes.c_int), ("eager", ctypes.c_int),
-                ("ehs_batch", ctypes.c_int)]
+                ("ehs_batch", ctypes.c_int), ("recorder", ctypes.c_void_p)]
 
 
 class GpuStats(ctypes.Structure):
@@ -210,6 +210,12 @@ class RequestStatus(ctypes.Structure):
                 ("message", ctypes.c_char * 256)]
 
 
+FRAME_DTYPE = np.dtype([("state", STATE_DTYPE), ("medium", np.int32),
+                        ("event", np.int32), ("history", np.uint64),
+                        ("reserved", np.uint64)])
+assert FRAME_DTYPE.itemsize == 128
+
+
 EXPORTED_SYMBOLS = (
     "pumas_gpu_context_default",
     "pumas_gpu_physics_upload",
@@ -225,6 +231,10 @@ EXPORTED_SYMBOLS = (
     "pumas_gpu_tally",
     "pumas_gpu_tally_host",
     "pumas_gpu_autotune",
+    "pumas_gpu_recorder_create",
+    "pumas_gpu_recorder_destroy",
+    "pumas_gpu_recorder_clear",
+    "pumas_gpu_recorder_read",
     "pumas_gpu_server_create",
     "pumas_gpu_server_destroy",
     "pumas_gpu_server_submit",
@@ -271,6 +281,16 @@ def declare(lib):
     lib.pumas_gpu_tally_host.argtypes = [vp, vp, ctypes.c_size_t,
                                          P(TallySpec), P(TallyResult)]
     lib.pumas_gpu_tally_host.restype = ctypes.c_int
+    lib.pumas_gpu_recorder_create.argtypes = [
+        ctypes.c_size_t, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, P(vp)]
+    lib.pumas_gpu_recorder_create.restype = ctypes.c_int
+    lib.pumas_gpu_recorder_destroy.argtypes = [P(vp)]
+    lib.pumas_gpu_recorder_destroy.restype = None
+    lib.pumas_gpu_recorder_clear.argtypes = [vp]
+    lib.pumas_gpu_recorder_clear.restype = ctypes.c_int
+    lib.pumas_gpu_recorder_read.argtypes = [vp, vp, ctypes.c_size_t,
+                                            P(ctypes.c_uint64)]
+    lib.pumas_gpu_recorder_read.restype = ctypes.c_int
     lib.pumas_gpu_autotune.argtypes = [
         vp, P(GpuContext), vp, vp, ctypes.c_size_t, ctypes.c_uint64,
         ctypes.c_int, P(GpuOptions), P(ctypes.c_double), vp]

@@ -75,7 +75,9 @@ enum {
         EVENT_VERTEX_COULOMB = 512,
         EVENT_VERTEX_DECAY = 1024,
         EVENT_VERTEX = 2016,
-        EVENT_WEIGHT = 2048
+        EVENT_WEIGHT = 2048,
+        EVENT_START = 4096,
+        EVENT_STOP = 8192
 };
 
 enum {

@@ -49,11 +49,39 @@ enum {
 struct Walker {
         Track t;
         int phase;
+        int record; /* frames of this history are recorded */
         int event, rc;
         int media0, end_medium;
         int medium, material, straight, step_index, step_max_type;
         double density, step_max_medium;
         double ti, wi, Xi, Xf, dei, grammage_max;
+};
+
+/* A recorded frame: layout of struct pumas_gpu_frame (include/pumas_gpu.h),
+ * the device counterpart of struct pumas_frame (include/pumas.h:386-395). */
+struct Frame {
+        double charge, energy, distance, grammage, time, weight;
+        double position[3], direction[3];
+        int decayed;
+        int pad;
+        int medium; /* index, -1 = outside */
+        int event;
+        uint64_t history;
+        uint64_t reserved;
+};
+
+/* Device frame recorder, as the reference's pumas_recorder
+ * (include/pumas.h:432-458): frames go to a device buffer in the order they
+ * are recorded (a counter is incremented per frame, frames beyond the
+ * capacity are counted and dropped), for histories with ids in
+ * [first, last). `period` has the reference's meaning: every period-th
+ * step, or only events and medium changes when zero or less. */
+struct Recorder {
+        Frame * frames; /* NULL: recording disabled */
+        unsigned long long * count;
+        unsigned long long capacity;
+        int period;
+        uint64_t first, last;
 };
 
 /*
@@ -70,6 +98,46 @@ template <bool F32> struct EngineT {
         const Config & c;
         const pg_geometry & g;
         const FloatTables & ft;
+        Recorder rec;
+
+        /* Records the state of `w` in `medium` with `event`, if the history
+         * is recorded (record_state, src/pumas.c:7738-7780). */
+        PG_NOINLINE void record(const Walker & w, const Track & t, int medium,
+            int event) const
+        {
+#ifdef __HIP_DEVICE_COMPILE__
+                const unsigned long long k = atomicAdd(rec.count, 1ull);
+#else
+                const unsigned long long k = (*rec.count)++;
+#endif
+                if (k >= rec.capacity) return;
+                Frame & f = rec.frames[k];
+                f.charge = t.charge;
+                f.energy = t.energy;
+                f.distance = t.distance;
+                f.grammage = t.grammage;
+                f.time = t.time;
+                f.weight = t.weight;
+                for (int j = 0; j < 3; j++) {
+                        f.position[j] = t.pos[j];
+                        f.direction[j] = t.dir[j];
+                }
+                f.decayed = t.decayed;
+                f.medium = medium;
+                f.event = event;
+                f.pad = 0;
+                f.history = ((uint64_t)t.rng.h1 << 32) | t.rng.h0;
+                f.reserved = 0;
+                (void)w;
+        }
+
+        /* The same, on a copy: keeps the Walker out of memory. */
+        PG_FN void record_frame(const Walker & w, int medium, int event) const
+        {
+                if (!w.record) return;
+                const Walker copy = w;
+                record(copy, copy.t, medium, event);
+        }
 
         /* Per-step table lookups, in the bulk precision. */
         PG_FN double bulk_grammage(Track & t, int scheme, int material,
@@ -1508,6 +1576,12 @@ template <bool F32> struct EngineT {
                 w.rc = RETURN_SUCCESS;
                 w.media0 = w.medium = -1;
                 w.phase = PHASE_DONE;
+                {
+                        const uint64_t id =
+                            ((uint64_t)t.rng.h1 << 32) | t.rng.h0;
+                        w.record = (rec.frames != nullptr) &&
+                            (id >= rec.first) && (id < rec.last);
+                }
                 if (t.decayed) {
                         w.event = EVENT_VERTEX_DECAY;
                         w.end_medium = -1;
@@ -1531,6 +1605,8 @@ template <bool F32> struct EngineT {
                 w.end_medium = -1;
                 if (medium < 0) {
                         w.event = EVENT_MEDIUM;
+                        record_frame(w, -1,
+                            EVENT_MEDIUM | EVENT_START | EVENT_STOP);
                         return;
                 } else if ((step_max_medium > 0.) &&
                     (step_max_type == STEP_CHECK))
@@ -1564,15 +1640,18 @@ template <bool F32> struct EngineT {
                  * is its start medium. */
                 w.end_medium = medium;
                 if (!do_stepping) {
+                        record_frame(w, medium, EVENT_START);
                         Track tmp = t;
                         const EngineT cold = *this;
                         w.event = cold.transport_with_csda(
                             tmp, material, density);
                         t = tmp;
+                        record_frame(w, medium, w.event | EVENT_STOP);
                         return;
                 }
 
                 /* transport_with_stepping set up, src/pumas.c:5210-5273. */
+                record_frame(w, medium, EVENT_START);
                 w.medium = medium;
                 w.material = material;
                 w.density = density;
@@ -1653,6 +1732,7 @@ template <bool F32> struct EngineT {
                 w.end_medium = w.medium;
                 w.event = t.step_event;
                 w.phase = PHASE_DONE;
+                record_frame(w, w.medium, t.step_event | EVENT_STOP);
         }
 
         /* Common tail of an event: new initial conditions and limits,
@@ -1703,7 +1783,11 @@ template <bool F32> struct EngineT {
                     w.grammage_max, w.step_max_medium, w.step_max_type,
                     &new_medium);
                 w.step_index++;
-                if (!t.step_event) return;
+                /* Recorded steps, src/pumas.c:5296-5328: every period-th
+                 * step also gets the event-time weight update. */
+                const int record_step = w.record && (rec.period > 0) &&
+                    (w.step_index != 1) && ((w.step_index % rec.period) == 0);
+                if (!t.step_event && !record_step) return;
 
                 if ((c.direction == MODE_BACKWARD) &&
                     (c.energy_loss >= MODE_CSDA)) {
@@ -1717,7 +1801,9 @@ template <bool F32> struct EngineT {
                         t.weight = w.wi * exp(-fabs(t.time - w.ti) / p.ctau);
                 }
 
-                if (t.step_event & (EVENT_LIMIT | EVENT_WEIGHT |
+                if (!t.step_event) {
+                        record_frame(w, w.medium, EVENT_NONE);
+                } else if (t.step_event & (EVENT_LIMIT | EVENT_WEIGHT |
                                        EVENT_VERTEX_DECAY)) {
                         walker_finish(w);
                 } else if (t.step_event & EVENT_VERTEX_DEL) {
@@ -1743,6 +1829,8 @@ template <bool F32> struct EngineT {
                             0.;
                         t.step_first = 1;
                         t.step_invlb1 = 0;
+                        if (!(c.event & EVENT_MEDIUM))
+                                record_frame(w, w.medium, t.step_event);
                         walker_post(w);
                 }
         }
@@ -1751,17 +1839,32 @@ template <bool F32> struct EngineT {
         PG_FN void walker_del(Walker & w) const
         {
                 Track & t = w.t;
+                /* Pre and post DEL points when recording, src/pumas.c:
+                 * 5343-5387: the pre point is the state after the DEL with
+                 * its energy and direction before it. */
+                const int rec_del = w.record && (rec.period > 0);
+                const double ki = t.energy;
+                const double ui[3] = { t.dir[0], t.dir[1], t.dir[2] };
                 {
                         Track tmp = t;
                         const EngineT cold = *this;
                         cold.transport_do_del(tmp, w.material);
                         t = tmp;
                 }
+                if (rec_del) {
+                        Walker pre = w;
+                        pre.t.energy = ki;
+                        pre.t.dir[0] = ui[0];
+                        pre.t.dir[1] = ui[1];
+                        pre.t.dir[2] = ui[2];
+                        record_frame(pre, w.medium, t.step_event);
+                }
                 if ((c.event & t.step_event) ||
                     t.step_event == EVENT_WEIGHT) {
                         walker_finish(w);
                         return;
                 }
+                if (rec_del) record_frame(w, w.medium, EVENT_NONE);
                 t.step_first = 1;
                 t.step_invlb1 = 0;
                 walker_post(w);

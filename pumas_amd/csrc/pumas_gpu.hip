@@ -49,6 +49,16 @@ extern "C" void pumas_gpu_context_default(struct pumas_gpu_context * context)
         context->accuracy = 1E-02;
 }
 
+/* Device frame buffer and counter of a recorder (see pumas_gpu_recorder_*
+ * below). */
+struct pumas_gpu_recorder {
+        Frame * frames;
+        unsigned long long * count;
+        unsigned long long capacity;
+        int period;
+        uint64_t first, n;
+};
+
 /* ------------------------------------------------------------------------
  * Physics upload.
  * ------------------------------------------------------------------------ */
@@ -551,6 +561,7 @@ struct KernelArgs {
         int32_t * rc;
         int max_steps;
         int ehs_batch;
+        Recorder rec;
         unsigned long long * counters;
         Walker * records;     /* [n] histories in flight */
         uint32_t * q_step[2]; /* [n] step queues (ping-pong) */
@@ -662,7 +673,7 @@ template <bool F32>
 __global__ void __launch_bounds__(256) k_init(KernelArgs args)
 {
         KERNEL_ARGS(args);
-        const EngineT<F32> eng = { a.p, a.mv, a.c, a.g, a.ft };
+        const EngineT<F32> eng = { a.p, a.mv, a.c, a.g, a.ft, a.rec };
         const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
         const bool active = i < a.n;
         Walker w;
@@ -696,7 +707,7 @@ template <bool F32>
 __global__ void __launch_bounds__(256, 2) k_step(KernelArgs args, int parity)
 {
         KERNEL_ARGS(args);
-        const EngineT<F32> eng = { a.p, a.mv, a.c, a.g, a.ft };
+        const EngineT<F32> eng = { a.p, a.mv, a.c, a.g, a.ft, a.rec };
         const uint64_t below = lanes_below();
         const uint32_t * in = a.q_step[parity];
         const unsigned long long n_in = a.counters[C_STEP_N0 + parity];
@@ -773,7 +784,7 @@ template <bool F32>
 __global__ void __launch_bounds__(256, 2) k_del(KernelArgs args, int parity)
 {
         KERNEL_ARGS(args);
-        const EngineT<F32> eng = { a.p, a.mv, a.c, a.g, a.ft };
+        const EngineT<F32> eng = { a.p, a.mv, a.c, a.g, a.ft, a.rec };
         const unsigned long long n = a.counters[C_DEL_N];
         const uint32_t * queue = a.q_del;
         const int lane = threadIdx.x & 63;
@@ -1055,6 +1066,15 @@ extern "C" enum pumas_return pumas_gpu_transport(
                                                             10000000;
         a.ehs_batch = (options && options->ehs_batch > 0) ? options->ehs_batch :
                                                             EHS_BATCH;
+        if (options && options->recorder) {
+                const pumas_gpu_recorder * r = options->recorder;
+                a.rec.frames = r->frames;
+                a.rec.count = r->count;
+                a.rec.capacity = r->capacity;
+                a.rec.period = r->period;
+                a.rec.first = r->first;
+                a.rec.last = r->first + r->n;
+        }
         if (a.ehs_batch > 64)
                 return pg_fail(PUMAS_RETURN_VALUE_ERROR,
                     "ehs_batch must be at most 64 (got %d)", a.ehs_batch);
@@ -1126,6 +1146,82 @@ extern "C" enum pumas_return pumas_gpu_transport(
         if (error != 0)
                 return pg_fail((enum pumas_return)error,
                     "a history raised error code %llu", error);
+        return PUMAS_RETURN_SUCCESS;
+}
+
+/* ------------------------------------------------------------------------
+ * Frame recorder.
+ * ------------------------------------------------------------------------ */
+static_assert(sizeof(Frame) == sizeof(struct pumas_gpu_frame),
+    "Frame and struct pumas_gpu_frame must have the same layout");
+
+extern "C" enum pumas_return pumas_gpu_recorder_create(size_t capacity,
+    int period, uint64_t first_history, uint64_t n_histories,
+    pumas_gpu_recorder ** recorder)
+{
+        pg_clear_error();
+        if (recorder == NULL)
+                return pg_fail(PUMAS_RETURN_VALUE_ERROR, "NULL argument");
+        *recorder = nullptr;
+        auto * r = new pumas_gpu_recorder();
+        r->capacity = capacity;
+        r->period = period;
+        r->first = first_history;
+        r->n = n_histories;
+        r->frames = nullptr;
+        r->count = nullptr;
+        hipError_t e = hipMalloc((void **)&r->count, sizeof(*r->count));
+        if (e == hipSuccess)
+                e = hipMalloc((void **)&r->frames,
+                    (capacity > 0 ? capacity : 1) * sizeof(Frame));
+        if (e == hipSuccess) e = hipMemset(r->count, 0x0, sizeof(*r->count));
+        if (e != hipSuccess) {
+                (void)hipFree(r->count);
+                (void)hipFree(r->frames);
+                delete r;
+                return pg_fail(PUMAS_RETURN_MEMORY_ERROR,
+                    "could not allocate the recorder (%s)",
+                    hipGetErrorString(e));
+        }
+        *recorder = r;
+        return PUMAS_RETURN_SUCCESS;
+}
+
+extern "C" void pumas_gpu_recorder_destroy(pumas_gpu_recorder ** recorder)
+{
+        if ((recorder == NULL) || (*recorder == NULL)) return;
+        (void)hipFree((*recorder)->frames);
+        (void)hipFree((*recorder)->count);
+        delete *recorder;
+        *recorder = nullptr;
+}
+
+extern "C" enum pumas_return pumas_gpu_recorder_clear(
+    pumas_gpu_recorder * recorder)
+{
+        pg_clear_error();
+        if (recorder == NULL)
+                return pg_fail(PUMAS_RETURN_VALUE_ERROR, "NULL recorder");
+        HIP_TRY(hipMemset(recorder->count, 0x0, sizeof(*recorder->count)));
+        return PUMAS_RETURN_SUCCESS;
+}
+
+extern "C" enum pumas_return pumas_gpu_recorder_read(
+    const pumas_gpu_recorder * recorder, struct pumas_gpu_frame * frames,
+    size_t max_frames, uint64_t * length)
+{
+        pg_clear_error();
+        if (recorder == NULL)
+                return pg_fail(PUMAS_RETURN_VALUE_ERROR, "NULL recorder");
+        unsigned long long count = 0;
+        HIP_TRY(hipMemcpy(&count, recorder->count, sizeof(count),
+            hipMemcpyDeviceToHost));
+        if (length) *length = count;
+        size_t k = (count < recorder->capacity) ? count : recorder->capacity;
+        if (k > max_frames) k = max_frames;
+        if ((k > 0) && frames)
+                HIP_TRY(hipMemcpy(frames, recorder->frames, k * sizeof(Frame),
+                    hipMemcpyDeviceToHost));
         return PUMAS_RETURN_SUCCESS;
 }
 

@@ -63,6 +63,16 @@ extern "C" int host_load(const char *path) {
     return 0;
 }
 
+static Recorder g_rec;
+
+/* Frames of the following host_transport calls go to `frames` (NULL stops
+ * recording), as the device recorder. */
+extern "C" void host_record(Frame *frames, unsigned long long *count,
+    unsigned long long capacity, int period, uint64_t first, uint64_t n) {
+    g_rec.frames = frames; g_rec.count = count; g_rec.capacity = capacity;
+    g_rec.period = period; g_rec.first = first; g_rec.last = first + n;
+}
+
 extern "C" int host_transport(const pumas_gpu_context *ctx, const pumas_gpu_geometry_desc *d,
     pumas_state *states, int64_t n, uint64_t seed, uint64_t first,
     int *events, int *media, uint32_t *steps, uint32_t *draws, int *rcs) {
@@ -77,7 +87,7 @@ extern "C" int host_transport(const pumas_gpu_context *ctx, const pumas_gpu_geom
     c.scattering = ctx->mode.scattering; c.event = ctx->event; c.limit_energy = ctx->limit.energy;
     c.limit_distance = ctx->limit.distance; c.limit_grammage = ctx->limit.grammage; c.limit_time = ctx->limit.time;
     c.accuracy = ctx->accuracy;
-    static FloatTables g_ft; Engine eng = {g_view, g_mv, c, g, g_ft};
+    static FloatTables g_ft; Engine eng = {g_view, g_mv, c, g, g_ft, g_rec};
     for (int64_t i = 0; i < n; i++) {
         pumas_state *s = states + i; Walker w; Track &t = w.t;
         t.charge = s->charge; t.energy = s->energy; t.distance = s->distance; t.grammage = s->grammage;

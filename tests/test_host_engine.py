@@ -44,6 +44,10 @@ def host():
              os.path.join(REPO, "include"), "-o", LIB, src])
     lib = ctypes.CDLL(LIB)
     lib.host_load.argtypes = [ctypes.c_char_p]
+    lib.host_record.argtypes = [ctypes.c_void_p, ctypes.c_void_p,
+                                ctypes.c_ulonglong, ctypes.c_int,
+                                ctypes.c_uint64, ctypes.c_uint64]
+    lib.host_record.restype = None
     lib.host_transport.argtypes = [
         ctypes.POINTER(_abi.GpuContext), ctypes.POINTER(_abi.GeometryDesc),
         ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64,
@@ -116,3 +120,36 @@ def test_voxels_bit_exact(host, ref_iron_aligned):
                         orc.RefGeometry(desc, aligned=True), states, seed=0,
                         threads=8)
     check_exact(out, ref, skip_aborted=True)
+
+
+@pytest.mark.parametrize("period", [1, 7, 0])
+@pytest.mark.parametrize("case", ["slabs_straggled", "slabs_backward",
+                                  "slabs_mixed", "csda_rock"])
+def test_recorder_frames_bit_exact(host, ref_standard_aligned, case, period):
+    """The device recorder's frames (track start and stop, every period-th
+    step, DEL pre and post points, medium changes) are the frames of the
+    reference's recorder, in order, bit for bit."""
+    import oracle as orc
+    name, pname, ctx, desc, states = next(
+        c for c in make_golden.cases() if c[0] == case)
+    states = states[:16]
+    cap = 1 << 16
+    frames = np.zeros(cap, _abi.FRAME_DTYPE)
+    count = ctypes.c_ulonglong(0)
+    host.host_record(frames.ctypes.data, ctypes.addressof(count), cap, period,
+                     0, len(states))
+    try:
+        out = run_host(host, pname, ctx, desc, states)
+    finally:
+        host.host_record(None, None, 0, 0, 0, 0)
+    fin, ref = orc.transport_record(ref_standard_aligned, ctx,
+                                    orc.RefGeometry(desc, aligned=True),
+                                    states, period=period, capacity=cap)
+    assert (out[0] == fin).all()
+    k = count.value
+    assert k == ref["count"] and k < cap and k >= 2 * len(states)
+    f = frames[:k]
+    assert (f["state"] == ref["states"]).all()
+    np.testing.assert_array_equal(f["medium"], ref["media"])
+    np.testing.assert_array_equal(f["event"], ref["events"])
+    np.testing.assert_array_equal(f["history"], ref["history"])
