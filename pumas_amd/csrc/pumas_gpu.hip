@@ -823,6 +823,8 @@ __global__ void zero_counters(unsigned long long * c, int n)
         if ((int)threadIdx.x < n) c[threadIdx.x] = 0ull;
 }
 
+size_t pg_bytes_per_history(void) { return sizeof(Walker) + 3 * sizeof(uint32_t); }
+
 /* ------------------------------------------------------------------------
  * C ABI transport.
  * ------------------------------------------------------------------------ */

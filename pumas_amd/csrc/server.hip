@@ -34,12 +34,13 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-/* Device bytes of a request in flight: the scheduler's history record
- * (336 B) and queue slots (12 B) per history, plus the state and output
- * copies of a host-memory request. */
+/* Device bytes of a request in flight: the scheduler's history record and
+ * queue slots per history, plus the state and output copies of a
+ * host-memory request. */
 size_t request_bytes(const struct pumas_gpu_request & r)
 {
-        const size_t per = 348 + (r.device ? 0 : 104 + 24);
+        const size_t per =
+            pg_bytes_per_history() + (r.device ? 0 : 104 + 24);
         return r.n * per + (size_t(1) << 20);
 }
 

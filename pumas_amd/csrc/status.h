@@ -14,6 +14,10 @@ enum pumas_return pg_fail(enum pumas_return rc, const char * fmt, ...);
 /* Clears the message at the start of an API call. */
 void pg_clear_error(void);
 
+/* Device bytes per history that the transport scheduler allocates (history
+ * record and queue slots). */
+size_t pg_bytes_per_history(void);
+
 #define HIP_TRY(expr)                                                          \
         do {                                                                   \
                 hipError_t e_ = (expr);                                        \
