@@ -16,7 +16,15 @@ per second are reported beside it. The CPU baseline is the reference library
 (compiled from its source by oracle/Makefile) with one context per thread on
 the host cores, over a bounded sample of the same workload.
 
+Other workloads (`--workload`, not the default line): `csda` — backward
+CSDA through 1 km of standard rock, 10^8 histories (closed form, one
+muon-step per history); `terrain` — backward straggled muography along 64
+lines of sight (10^3..10^6 events each) under a 1024 x 1024 heightmap;
+`voxels` — backward straggled transport through 512^3 voxels of 2 m at a 1%
+cutoff. `--precision fp32` selects FP32-bulk.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N]
+                       [--workload slabs|csda|terrain|voxels]
 """
 import argparse
 import json
@@ -31,6 +39,8 @@ sys.path.insert(0, HERE)
 # struct pumas_state (104 B) plus the per-history stepping carry (96 B),
 # read and written once per step.
 BYTES_PER_STEP_FP64 = 2 * (104 + 96)
+# Closed form CSDA path: the state, read and written once per history.
+BYTES_PER_HISTORY_CSDA = 2 * 104
 HBM_PEAK_GBS = 8000.0
 
 
@@ -43,8 +53,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=10_000_000,
-                    help="histories per pass (the workload has 10^7)")
+    ap.add_argument("--workload", default="slabs",
+                    choices=("slabs", "csda", "terrain", "voxels"))
+    ap.add_argument("--n", type=int, default=0,
+                    help="histories per pass (0: the workload's own, 10^7 "
+                         "for slabs and voxels, 10^8 for csda)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target duration of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -83,17 +96,15 @@ def copy_calibration(mib):
                       "launches": 3}))
 
 
-def cpu_baseline(states, seconds):
+def cpu_baseline(states, seconds, physics, desc, ctx):
     """Reference CPU path, one context per thread, on a bounded sample."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import oracle as orc
-    from pumas_amd import workloads as W
     if not orc.available():
         return None
     threads = min(16, len(os.sched_getaffinity(0)))
-    phys = orc.RefPhysics("standard_c005")
-    geom = orc.RefGeometry(W.slabs_geometry())
-    ctx = W.slabs_forward_context()
+    phys = orc.RefPhysics(physics)
+    geom = orc.RefGeometry(desc)
     # Calibrate on a small sample, then size the sample to `seconds`.
     n = 256 * threads
     r = orc.transport(phys, ctx, geom, states[:n], seed=0, threads=threads)
@@ -168,6 +179,45 @@ def serving(pa, phys, seconds, slots):
     }
 
 
+WORKLOADS = {
+    "slabs": "forward straggled + mixed scattering, 3 slabs (100 m air / "
+             "500 m standard rock / 200 m water), K0 log-uniform [10, 1e4] "
+             "GeV, cos(zenith) in [0.5, 1]",
+    "csda": "backward CSDA, no scattering, 1 km of standard rock, K_f "
+            "log-uniform [1e-2, 1e2] GeV",
+    "terrain": "backward straggled + mixed scattering under a 1024 x 1024 "
+               "heightmap (5 m, relief 0-800 m, column density 2.0-2.9), 64 "
+               "lines of sight x 10^U[3,6] events, K_f log-uniform "
+               "[1e-2, 1e2] GeV",
+    "voxels": "backward straggled + mixed scattering through 512^3 voxels "
+              "of 2 m (rock / water / air / iron, lognormal density), "
+              "cutoff 0.01, K_f log-uniform [0.1, 1e3] GeV",
+}
+
+
+def setup(args, pa, W):
+    """(physics name, geometry description, context, host states, n)."""
+    wl = args.workload
+    if wl == "slabs":
+        n = args.n or 10_000_000
+        log(f"bench: generating {n} initial states")
+        return ("standard_c005", W.slabs_geometry(),
+                W.slabs_forward_context(), W.slabs_forward_states(n), n)
+    if wl == "csda":
+        n = args.n or 100_000_000
+        return ("standard_c005", W.csda_rock_geometry(),
+                W.csda_rock_context(), None, n)
+    if wl == "terrain":
+        log("bench: generating the 64 lines of sight")
+        states, _ = W.terrain_sightlines(64, 1023 * 5.0)
+        return ("standard_c005", W.terrain_geometry(), W.terrain_context(),
+                states, len(states))
+    n = args.n or 10_000_000
+    log(f"bench: generating the 512^3 voxel grid and {n} states")
+    return ("iron_c001", W.voxel_geometry_device(), W.voxel_context(),
+            W.voxel_states(n, 512 * 2.0), n)
+
+
 def main():
     args = parse()
     if args.copy_mib > 0:
@@ -184,24 +234,32 @@ def main():
     import pumas_amd as pa
     from pumas_amd import workloads as W
 
-    n = args.n
-    log(f"bench: generating {n} initial states")
-    states = W.slabs_forward_states(n, seed=0)
-    phys = pa.Physics(pa.physics_path("standard_c005"))
-    geom = pa.Geometry(phys, W.slabs_geometry())
-    ctx = W.slabs_forward_context()
-
+    pname, desc, ctx, states, n = setup(args, pa, W)
+    phys = pa.Physics(pa.physics_path(pname))
+    geom = pa.Geometry(phys, desc)
     dev = torch.device("cuda:0")
-    pristine = torch.from_numpy(states.view(np.uint8).reshape(-1)).to(dev)
+    if states is None:
+        # CSDA: 10^7 generated states tiled on the device (the closed form
+        # path draws no random numbers; the copies are independent work).
+        base = W.csda_rock_states(min(n, 10_000_000))
+        t = torch.from_numpy(base.view(np.uint8).reshape(-1, 104)).to(dev)
+        reps = (n + len(base) - 1) // len(base)
+        pristine = t.repeat(reps, 1)[:n].reshape(-1).contiguous()
+        sample = base
+    else:
+        pristine = torch.from_numpy(states.view(np.uint8).reshape(-1)).to(dev)
+        sample = states
     work = torch.empty_like(pristine)
     stream = torch.cuda.current_stream().cuda_stream
+    csda = args.workload == "csda"
 
     precision = pa.Precision.FP64 if args.precision == "fp64" else \
         pa.Precision.FP32_BULK
 
     tuned = {"block_size": args.block_size,
              "blocks_per_cu": args.blocks_per_cu, "ehs_batch": 0}
-    if not args.no_tune and args.block_size == 0 and args.blocks_per_cu == 0:
+    if (not args.no_tune and not csda and args.block_size == 0 and
+            args.blocks_per_cu == 0):
         m = min(n, args.tune_n)
         tuned, rate = pa.autotune(phys, ctx, geom, pristine.data_ptr(), m,
                                   seed=7, precision=precision, stream=stream)
@@ -233,34 +291,45 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
 
-    total_steps = sum(st.steps for st in passes)
-    steps_per_s = total_steps / elapsed
     events_per_s = args.steps * n / elapsed
     mean_ms = sum(st.kernel_ms for st in passes) / len(passes)
-    # Roofline of the dominant kernel (the stepping kernel): algorithmic
-    # bytes of its launches over their summed duration (HIP events around
-    # each launch, on the stream it runs on).
-    step_ms = sum(st.step_kernel_ms for st in passes)
-    launches = sum(st.step_launches for st in passes)
-    if step_ms <= 0.0:  # graph replays: no per-launch timing
-        step_ms = sum(st.kernel_ms for st in passes)
-    achieved = total_steps * BYTES_PER_STEP_FP64 / (step_ms * 1e-3) / 1e9
+    if csda:
+        # One muon-step per history on the closed form path; the dominant
+        # kernel is k_init (one launch per chunk), timed with the call.
+        total_steps = args.steps * n
+        kernel, launches = "k_init", args.steps * ((n + (1 << 25) - 1) >>
+                                                   25)
+        kernel_ms = sum(st.kernel_ms for st in passes)
+        unit_bytes = BYTES_PER_HISTORY_CSDA
+    else:
+        # Roofline of the stepping kernel: algorithmic bytes of its
+        # launches over their summed duration (HIP events around each
+        # launch, on the stream it runs on).
+        total_steps = sum(st.steps for st in passes)
+        kernel, launches = "k_step", sum(st.step_launches for st in passes)
+        kernel_ms = sum(st.step_kernel_ms for st in passes)
+        if kernel_ms <= 0.0:  # graph replays: no per-launch timing
+            kernel_ms = sum(st.kernel_ms for st in passes)
+        unit_bytes = BYTES_PER_STEP_FP64
+    steps_per_s = total_steps / elapsed
+    achieved = total_steps * unit_bytes / (kernel_ms * 1e-3) / 1e9
 
     traffic = None
     tpath = os.path.join(HERE, "profiles", "k_step_slabs_traffic.json")
-    if os.path.exists(tpath) and n == 10_000_000:
+    if (args.workload == "slabs" and os.path.exists(tpath) and
+            n == 10_000_000):
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
 
     serve = None
-    if args.serve_seconds > 0:
+    if args.serve_seconds > 0 and args.workload == "slabs":
         log("bench: serving run")
         serve = serving(pa, phys, args.serve_seconds, args.serve_slots)
 
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline (reference, all host cores)")
-        cpu = cpu_baseline(states, args.cpu_seconds)
+        cpu = cpu_baseline(sample, args.cpu_seconds, pname, desc, ctx)
 
     line = {
         "metric": "muon-steps/s & events/s at <=0.1% flux bias vs FP64; "
@@ -278,10 +347,7 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {
-            "workload": "forward straggled + mixed scattering, 3 slabs "
-                        "(100 m air / 500 m standard rock / 200 m water), "
-                        "K0 log-uniform [10, 1e4] GeV, cos(zenith) in "
-                        "[0.5, 1]",
+            "workload": WORKLOADS[args.workload],
             "histories": n,
             "precision": args.precision,
             "launch": tuned,
@@ -293,10 +359,10 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "k_step",
-            "kernel_avg_ms": step_ms / max(launches, 1),
+            "kernel": kernel,
+            "kernel_avg_ms": kernel_ms / max(launches, 1),
             "launches_per_pass": launches / args.steps,
-            "bytes_per_step": BYTES_PER_STEP_FP64,
+            "bytes_per_unit": unit_bytes,
         },
         "transport_ms_per_pass": mean_ms,
         "cpu_baseline": cpu,

@@ -393,3 +393,89 @@ def serving_inputs(req, terrain_extent, seed=0):
     ctx = terrain_context().replace(energy_loss=req.energy_loss)
     return ctx, terrain_states(req.n, terrain_extent, seed=seed,
                                first=req.first)
+
+
+# ---------------------------------------------------------------------------
+# Full-size inputs of the terrain and voxel workloads.
+# ---------------------------------------------------------------------------
+def terrain_sightlines(n_lines, extent, seed=0, nmin=1e3, nmax=1e6,
+                       kmin=1e-2, kmax=1e2, cmin=0.5, cmax=1.0):
+    """Batched muography: `n_lines` lines of sight, each a detector under
+    the terrain (uniform in the central half of the map) looking up along a
+    direction in the cone cos(zenith) in [cmin, cmax], with
+    round(10^U[log nmin, log nmax]) events of final kinetic energy
+    log-uniform in [kmin, kmax] GeV. Returns (states, line index per state);
+    history ids are consecutive."""
+    lines = np.arange(n_lines, dtype=np.uint64)
+    v = [uniform(seed, 6, lines, k) for k in range(5)]
+    counts = np.rint(10 ** (math.log10(nmin) + (math.log10(nmax) -
+                                                  math.log10(nmin)) * v[0]))
+    counts = counts.astype(np.int64)
+    n = int(counts.sum())
+    line = np.repeat(np.arange(n_lines), counts)
+    ids = _ids(n, 0)
+    u0 = uniform(seed, 0, ids, 0)
+    rk = math.log(kmax / kmin)
+    s = np.zeros(n, STATE_DTYPE)
+    s["charge"] = -1.0
+    s["energy"] = kmin * np.exp(rk * u0)
+    s["weight"] = s["energy"] * rk
+    ct = (cmin + (cmax - cmin) * v[1])[line]
+    st = np.sqrt(1.0 - ct * ct)
+    phi = (2.0 * math.pi * v[2])[line]
+    s["direction"][:, 0] = st * np.cos(phi)
+    s["direction"][:, 1] = st * np.sin(phi)
+    s["direction"][:, 2] = -ct
+    s["position"][:, 0] = (0.5 * extent * (v[3] - 0.5))[line]
+    s["position"][:, 1] = (0.5 * extent * (v[4] - 0.5))[line]
+    s["position"][:, 2] = TERRAIN_DETECTOR_Z
+    return s, line
+
+
+def voxel_geometry_device(n=512, size=2.0, seed=0, sigma=0.1):
+    """voxel_geometry at full size (512^3), generated on the GPU with torch's
+    seeded generator: the same noise construction, thresholds and lognormal
+    density scale, other random numbers. For throughput runs; the parity
+    tests use voxel_geometry."""
+    import torch
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    out = torch.zeros((n, n, n), dtype=torch.float32, device=dev)
+    amp, total = 1.0, 0.0
+    f_base = torch.linspace(0, 1, n, device=dev, dtype=torch.float64)
+    for o in range(4):
+        cells = 2 * 2 ** o
+        lat = torch.rand((cells + 1,) * 3, generator=g, device=dev,
+                         dtype=torch.float64)
+        f = f_base * cells
+        i = torch.clamp(f.long(), max=cells - 1)
+        h = (f - i).float()
+        acc = torch.zeros_like(out)
+        for dz in (0, 1):
+            wz = (h if dz else 1 - h)[:, None, None]
+            for dy in (0, 1):
+                wy = (h if dy else 1 - h)[None, :, None]
+                for dx in (0, 1):
+                    wx = (h if dx else 1 - h)[None, None, :]
+                    v = lat[(i + dz)[:, None, None], (i + dy)[None, :, None],
+                            (i + dx)[None, None, :]].float()
+                    acc += wz * wy * wx * v
+        out += amp * acc
+        total += amp
+        amp *= 0.5
+    out /= total
+    label = torch.zeros((n, n, n), dtype=torch.uint8, device=dev)
+    label[out < 0.45] = 1
+    label[out < 0.35] = 2
+    label[out >= 0.7] = 3
+    del out
+    rho0 = torch.tensor(VOXEL_DEFAULT_DENSITY, dtype=torch.float32,
+                        device=dev)
+    scale = torch.exp(sigma * torch.randn((n, n, n), generator=g, device=dev)
+                      - 0.5 * sigma * sigma)
+    rho = rho0[label.long()] * scale
+    half = 0.5 * n * size
+    return GeometryDescription.voxels(
+        label.cpu().numpy(), rho.cpu().numpy(), VOXEL_MATERIALS, x0=-half,
+        y0=-half, z0=-half, dx=size, dy=size, dz=size)
