@@ -57,7 +57,7 @@ def parse():
                     choices=("slabs", "csda", "terrain", "voxels"))
     ap.add_argument("--n", type=int, default=0,
                     help="histories per pass (0: the workload's own, 10^7 "
-                         "for slabs and voxels, 10^8 for csda)")
+                         "for slabs, 2 10^6 for voxels, 10^8 for csda)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target duration of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -212,9 +212,9 @@ def setup(args, pa, W):
         states, _ = W.terrain_sightlines(64, 1023 * 5.0)
         return ("standard_c005", W.terrain_geometry(), W.terrain_context(),
                 states, len(states))
-    n = args.n or 10_000_000
+    n = args.n or 2_000_000
     log(f"bench: generating the 512^3 voxel grid and {n} states")
-    return ("iron_c001", W.voxel_geometry_device(), W.voxel_context(),
+    return ("iron_c001", W.voxel_geometry_large(), W.voxel_context(),
             W.voxel_states(n, 512 * 2.0), n)
 
 

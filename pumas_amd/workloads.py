@@ -432,22 +432,20 @@ def terrain_sightlines(n_lines, extent, seed=0, nmin=1e3, nmax=1e6,
     return s, line
 
 
-def voxel_geometry_device(n=512, size=2.0, seed=0, sigma=0.1):
-    """voxel_geometry at full size (512^3), generated on the GPU with torch's
-    seeded generator: the same noise construction, thresholds and lognormal
-    density scale, other random numbers. For throughput runs; the parity
-    tests use voxel_geometry."""
+def voxel_geometry_large(n=512, size=2.0, seed=0, sigma=0.1):
+    """voxel_geometry at full size (512^3), generated with torch's seeded CPU
+    generator (vectorised, reproducible on any host): the same noise
+    construction, thresholds and lognormal density scale, other random
+    numbers. For throughput runs; the parity tests use voxel_geometry."""
     import torch
-    dev = torch.device("cuda:0")
-    g = torch.Generator(device=dev)
+    g = torch.Generator()
     g.manual_seed(seed)
-    out = torch.zeros((n, n, n), dtype=torch.float32, device=dev)
+    out = torch.zeros((n, n, n), dtype=torch.float32)
     amp, total = 1.0, 0.0
-    f_base = torch.linspace(0, 1, n, device=dev, dtype=torch.float64)
+    f_base = torch.linspace(0, 1, n, dtype=torch.float64)
     for o in range(4):
         cells = 2 * 2 ** o
-        lat = torch.rand((cells + 1,) * 3, generator=g, device=dev,
-                         dtype=torch.float64)
+        lat = torch.rand((cells + 1,) * 3, generator=g, dtype=torch.float64)
         f = f_base * cells
         i = torch.clamp(f.long(), max=cells - 1)
         h = (f - i).float()
@@ -465,17 +463,16 @@ def voxel_geometry_device(n=512, size=2.0, seed=0, sigma=0.1):
         total += amp
         amp *= 0.5
     out /= total
-    label = torch.zeros((n, n, n), dtype=torch.uint8, device=dev)
+    label = torch.zeros((n, n, n), dtype=torch.uint8)
     label[out < 0.45] = 1
     label[out < 0.35] = 2
     label[out >= 0.7] = 3
     del out
-    rho0 = torch.tensor(VOXEL_DEFAULT_DENSITY, dtype=torch.float32,
-                        device=dev)
-    scale = torch.exp(sigma * torch.randn((n, n, n), generator=g, device=dev)
-                      - 0.5 * sigma * sigma)
+    rho0 = torch.tensor(VOXEL_DEFAULT_DENSITY, dtype=torch.float32)
+    scale = torch.exp(sigma * torch.randn((n, n, n), generator=g) -
+                      0.5 * sigma * sigma)
     rho = rho0[label.long()] * scale
     half = 0.5 * n * size
     return GeometryDescription.voxels(
-        label.cpu().numpy(), rho.cpu().numpy(), VOXEL_MATERIALS, x0=-half,
-        y0=-half, z0=-half, dx=size, dy=size, dz=size)
+        label.numpy(), rho.numpy(), VOXEL_MATERIALS, x0=-half, y0=-half,
+        z0=-half, dx=size, dy=size, dz=size)
