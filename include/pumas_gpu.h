@@ -263,6 +263,11 @@ struct pumas_gpu_options {
         int ehs_batch;
         /* Frame recorder, or NULL (see below). */
         const struct pumas_gpu_recorder * recorder;
+        /* DELs of the last histories of a batch applied inside the
+         * stepping wavefronts instead of in queued rounds: > 0 always,
+         * < 0 never, 0 for batches below 2^22 histories (latency of small
+         * requests). */
+        int tail_del;
 };
 
 /*

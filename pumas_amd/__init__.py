@@ -291,7 +291,8 @@ class TransportStats:
 
 
 def _options(precision, block_size, blocks_per_cu, max_steps,
-             rounds_per_check=0, eager=False, ehs_batch=0, recorder=None):
+             rounds_per_check=0, eager=False, ehs_batch=0, recorder=None,
+             tail_del=0):
     o = _abi.GpuOptions()
     o.precision = int(precision)
     o.block_size = block_size
@@ -301,13 +302,15 @@ def _options(precision, block_size, blocks_per_cu, max_steps,
     o.eager = int(bool(eager))
     o.ehs_batch = ehs_batch
     o.recorder = recorder.handle if recorder is not None else None
+    o.tail_del = int(tail_del)
     return o
 
 
 def transport(physics, context, geometry, states, seed=0, first_history=0,
               outputs=True, precision=Precision.FP64, block_size=0,
               blocks_per_cu=0, max_steps=0, raise_errors=True,
-              rounds_per_check=0, eager=False, ehs_batch=0, recorder=None):
+              rounds_per_check=0, eager=False, ehs_batch=0, recorder=None,
+              tail_del=0):
     """Transport host `states` (a STATE_DTYPE array, updated in place).
 
     Batch counterpart of pumas_context_transport: history i draws from the
@@ -335,7 +338,7 @@ def transport(physics, context, geometry, states, seed=0, first_history=0,
         out.rc = res.rc.ctypes.data
     stats = _abi.GpuStats()
     opts = _options(precision, block_size, blocks_per_cu, max_steps,
-                    rounds_per_check, eager, ehs_batch, recorder)
+                    rounds_per_check, eager, ehs_batch, recorder, tail_del)
     ctx = context.to_c()
     rc = lib.pumas_gpu_transport_host(
         physics.handle, ctypes.byref(ctx), geometry.handle,
@@ -353,7 +356,7 @@ def transport_device(physics, context, geometry, states_ptr, n, seed=0,
                      first_history=0, outputs=None, precision=Precision.FP64,
                      block_size=0, blocks_per_cu=0, max_steps=0, stream=None,
                      stats=True, rounds_per_check=0, eager=False,
-                     ehs_batch=0, recorder=None):
+                     ehs_batch=0, recorder=None, tail_del=0):
     """Transport `n` states already in device memory at `states_ptr`.
 
     `outputs` maps any of "events", "media", "steps", "draws", "rc" to a
@@ -366,7 +369,7 @@ def transport_device(physics, context, geometry, states_ptr, n, seed=0,
         setattr(out, k, v)
     st = _abi.GpuStats()
     opts = _options(precision, block_size, blocks_per_cu, max_steps,
-                    rounds_per_check, eager, ehs_batch, recorder)
+                    rounds_per_check, eager, ehs_batch, recorder, tail_del)
     ctx = context.to_c()
     _check(lib.pumas_gpu_transport(
         physics.handle, ctypes.byref(ctx), geometry.handle,

@@ -147,7 +147,8 @@ class GpuOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("block_size", ctypes.c_int),
                 ("blocks_per_cu", ctypes.c_int), ("max_steps", ctypes.c_int),
                 ("rounds_per_check", ctypes.c_int), ("eager", ctypes.c_int),
-                ("ehs_batch", ctypes.c_int), ("recorder", ctypes.c_void_p)]
+                ("ehs_batch", ctypes.c_int), ("recorder", ctypes.c_void_p),
+                ("tail_del", ctypes.c_int)]
 
 
 class GpuStats(ctypes.Structure):

@@ -702,8 +702,16 @@ __global__ void k_round(unsigned long long * c, int parity)
         }
 }
 
-/* Stepping: persistent over the step queue of round `parity`. */
-template <bool F32>
+/*
+ * Stepping: persistent over the step queue of round `parity`. With
+ * TAIL_DEL, a wavefront that can no longer refill (the step queue is
+ * exhausted) applies the DELs of its histories itself instead of queueing
+ * them for the next round: the last histories of a batch then run to their
+ * end without a round barrier per DEL, which sets the latency of small
+ * batches. The out-of-line DEL call costs registers on every step, so large
+ * batches use the variant without it.
+ */
+template <bool F32, bool TAIL_DEL>
 __global__ void __launch_bounds__(256, 2) k_step(KernelArgs args, int parity)
 {
         KERNEL_ARGS(args);
@@ -720,7 +728,8 @@ __global__ void __launch_bounds__(256, 2) k_step(KernelArgs args, int parity)
         for (;;) {
                 /* Hand over the histories that left the stepping phase. */
                 const bool done = w.phase == PHASE_DONE;
-                const bool to_del = w.phase == PHASE_DEL;
+                const bool to_del =
+                    (w.phase == PHASE_DEL) && !(TAIL_DEL && exhausted);
                 if (done) {
                         walker_store(a, slot, w);
                         steps_acc += w.t.steps;
@@ -754,7 +763,8 @@ __global__ void __launch_bounds__(256, 2) k_step(KernelArgs args, int parity)
                         }
                 }
                 if (!__ballot((w.phase == PHASE_STEP) ||
-                        (w.phase == PHASE_EHS))) {
+                        (w.phase == PHASE_EHS) ||
+                        (TAIL_DEL && (w.phase == PHASE_DEL)))) {
                         if (exhausted) break;
                         continue;
                 }
@@ -774,6 +784,8 @@ __global__ void __launch_bounds__(256, 2) k_step(KernelArgs args, int parity)
                 if (ehs && ((waiting >= batch) || !stepping)) {
                         if (w.phase == PHASE_EHS) eng.walker_ehs(w);
                 }
+                if (TAIL_DEL && exhausted && (w.phase == PHASE_DEL))
+                        eng.walker_del(w);
         }
         steps_flush(a, steps_acc);
 }
@@ -834,6 +846,10 @@ namespace {
  * bytes of HBM (about 11 GB). */
 constexpr size_t MAX_CHUNK = size_t(1) << 25;
 
+/* Chunks below this many histories step with in-wavefront DELs once their
+ * step queue is exhausted (k_step<F32, true>). */
+constexpr size_t TAIL_DEL_MAX = size_t(1) << 22;
+
 /* Default number of rounds launched between two reads of the queue
  * lengths, and its maximum. */
 constexpr int ROUNDS_PER_CHECK = 16;
@@ -843,6 +859,7 @@ constexpr int MAX_ROUNDS_PER_CHECK = 256;
 struct Launch {
         int n_cu, block, per_cu, rounds_per_check;
         bool graph;
+        int tail_del; /* <0 never, >0 always, 0: below TAIL_DEL_MAX */
         unsigned step_grid, del_grid;
         hipStream_t stream;
         hipEvent_t * marks; /* [2 * rounds_per_check] or NULL */
@@ -856,8 +873,12 @@ enum pumas_return launch_round(
         hipLaunchKernelGGL(k_round, dim3(1), dim3(64), 0, stream, a.counters,
             parity);
         if (L.marks) HIP_TRY(hipEventRecord(L.marks[2 * r], stream));
-        hipLaunchKernelGGL(k_step<F32>, dim3(L.step_grid), dim3(L.block), 0,
-            stream, a, parity);
+        if (L.tail_del)
+                hipLaunchKernelGGL((k_step<F32, true>), dim3(L.step_grid),
+                    dim3(L.block), 0, stream, a, parity);
+        else
+                hipLaunchKernelGGL((k_step<F32, false>), dim3(L.step_grid),
+                    dim3(L.block), 0, stream, a, parity);
         if (L.marks) HIP_TRY(hipEventRecord(L.marks[2 * r + 1], stream));
         hipLaunchKernelGGL(k_del<F32>, dim3(L.del_grid), dim3(256), 0,
             stream, a, parity);
@@ -882,6 +903,8 @@ enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
         L.del_grid = (unsigned)((del_fill < del_max) ?
                 (del_fill > 0 ? del_fill : 1) :
                 del_max);
+        L.tail_del = (L.tail_del > 0) ||
+            ((L.tail_del == 0) && (m < TAIL_DEL_MAX));
         hipStream_t stream = L.stream;
         KernelArgs a = a0;
         a.states = a0.states + base;
@@ -1124,8 +1147,10 @@ extern "C" enum pumas_return pumas_gpu_transport(
         }
         /* Per-launch timing of the stepping kernel needs eager launches:
          * events recorded inside a graph cannot be timed. */
-        const Launch L = { n_cu, block, per_cu, rounds_per_check, graph, 0, 0,
-                stream, (stats && !graph) ? marks.data() : nullptr };
+        const int tail_del = options ? options->tail_del : 0;
+        const Launch L = { n_cu, block, per_cu, rounds_per_check, graph,
+                tail_del, 0, 0, stream,
+                (stats && !graph) ? marks.data() : nullptr };
         for (size_t base = 0; base < n; base += MAX_CHUNK) {
                 const size_t m = (n - base < MAX_CHUNK) ? n - base : MAX_CHUNK;
                 rc = transport_chunk(

@@ -224,19 +224,20 @@ def test_backward_weights_vs_aligned_reference(ref_standard_aligned,
     assert_parity(gpu, ref, RTOL_STEPPING)
 
 
-@pytest.mark.parametrize("block_size,blocks_per_cu,ehs_batch",
-                         [(64, 8, 8), (128, 4, 32), (256, 1, 1)])
+@pytest.mark.parametrize("block_size,blocks_per_cu,ehs_batch,tail_del",
+                         [(64, 8, 8, 0), (128, 4, 32, 1), (256, 1, 1, -1),
+                          (256, 2, 16, -1)])
 def test_launch_settings_invariance(gpu_standard, block_size, blocks_per_cu,
-                                    ehs_batch):
-    """Launch shape and EHS batching change the interleaving of histories,
-    never their results."""
+                                    ehs_batch, tail_del):
+    """Launch shape, EHS batching and in-wavefront tail DELs change the
+    interleaving of histories, never their results."""
     s = W.slabs_forward_states(4000, seed=13)
     geom = pa.Geometry(gpu_standard, W.slabs_geometry())
     ctx = W.slabs_forward_context()
     ref = pa.transport(gpu_standard, ctx, geom, s.copy(), seed=13)
     alt = pa.transport(gpu_standard, ctx, geom, s.copy(), seed=13,
                        block_size=block_size, blocks_per_cu=blocks_per_cu,
-                       ehs_batch=ehs_batch)
+                       ehs_batch=ehs_batch, tail_del=tail_del)
     assert (alt.states == ref.states).all()
     for k in ("events", "media", "steps", "draws", "rc"):
         np.testing.assert_array_equal(getattr(alt, k), getattr(ref, k))
