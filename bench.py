@@ -70,6 +70,9 @@ def parse():
                     help="skip the autotuner (default launch settings)")
     ap.add_argument("--tune-n", type=int, default=2_000_000,
                     help="histories of the autotuner's sample")
+    ap.add_argument("--lds-tables", type=int, default=0,
+                    help="FP32 tables of the stepping kernel in LDS: 0 when "
+                         "they fit, -1 never")
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--serve-seconds", type=float, default=0.25,
@@ -272,7 +275,8 @@ def main():
             first_history=k * n, stream=stream, precision=precision,
             block_size=tuned["block_size"],
             blocks_per_cu=tuned["blocks_per_cu"],
-            ehs_batch=tuned["ehs_batch"], eager=not args.graph)
+            ehs_batch=tuned["ehs_batch"], eager=not args.graph,
+            lds_tables=args.lds_tables)
 
     for k in range(args.warmup):
         st = one_pass(1000 + k)

@@ -268,6 +268,12 @@ struct pumas_gpu_options {
          * < 0 never, 0 for batches below 2^22 histories (latency of small
          * requests). */
         int tail_del;
+        /* Single precision tables of the FP32-bulk stepping kernel read
+         * from a copy in each block's LDS when they fit beside the other
+         * resident blocks of a CU: 0 or > 0 when they fit, < 0 never (the
+         * tables are then read from HBM through the caches). Results do not
+         * depend on it. */
+        int lds_tables;
 };
 
 /*

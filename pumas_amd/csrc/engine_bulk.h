@@ -11,13 +11,17 @@
 
 namespace pg {
 
-struct FloatTables {
+/* The single precision tables, reached through pointers of type P: plain
+ * pointers to HBM, or pointers to a copy in LDS (see the stepping kernel). */
+template <class P> struct FloatTablesT {
         int n_energies, n_materials;
         float mass;
-        const float *K, *X, *X_dK, *dE, *dE_dK, *Omega, *Omega_dK;
-        const float *CS, *CS_dK, *Mu0, *Mu0_dK, *Ms1, *Ms1_dK, *Lb;
-        const float *a_max, *b_max;
+        P K, X, X_dK, dE, dE_dK, Omega, Omega_dK;
+        P CS, CS_dK, Mu0, Mu0_dK, Ms1, Ms1_dK, Lb;
+        P a_max, b_max;
 };
+
+using FloatTables = FloatTablesT<const float *>;
 
 /* Bracketing rows memory for float grids (see Cache). */
 struct FloatCache {
@@ -33,7 +37,8 @@ struct FloatCache {
 /* Bracketing row of a value missed by the first cached row: second cached
  * row and neighbours of the first one, then bisection (as
  * cached_index_miss). */
-PG_FN int findex_miss(const float * table, int n, float value, int * last)
+template <class P>
+PG_FN int findex_miss(P table, int n, float value, int * last)
 {
         const int i0 = last[0];
         const float below = table[(i0 > 0) ? i0 - 1 : 0];
@@ -77,8 +82,9 @@ PG_FN float fpchip(float t, float p0, float p1, float m0, float m1)
 
 /* Interpolation with speculative rows at the first cached bracket (one
  * round trip to memory on a cache hit), as interpolate. */
-PG_FN float finterpolate(const FloatTables & f, int * last,
-    const float * tx, const float * ty, const float * tm, float x)
+template <class P>
+PG_FN float finterpolate(const FloatTablesT<P> & f, int * last, P tx, P ty,
+    P tm, float x)
 {
         int i1 = last[0];
         float x0 = tx[i1], x1 = tx[i1 + 1];
@@ -100,24 +106,26 @@ PG_FN float finterpolate(const FloatTables & f, int * last,
         return fpchip(t, y0, y1, m1, m2);
 }
 
-PG_FN const float * fcol2(
-    const FloatTables & f, const float * t, int scheme, int material)
+template <class P>
+PG_FN P fcol2(const FloatTablesT<P> & f, P t, int scheme, int material)
 {
         return t + (scheme2(scheme) * f.n_materials + material) * f.n_energies;
 }
 
-PG_FN const float * fcol1(const FloatTables & f, const float * t, int material)
+template <class P>
+PG_FN P fcol1(const FloatTablesT<P> & f, P t, int material)
 {
         return t + material * f.n_energies;
 }
 
 /* CSDA range, as cel_grammage. */
-PG_FN float fcel_grammage(const FloatTables & f, FloatCache & ch, int scheme,
+template <class P>
+PG_FN float fcel_grammage(const FloatTablesT<P> & f, FloatCache & ch, int scheme,
     int material, float kinetic)
 {
         const int imax = f.n_energies - 1;
         if (kinetic < f.K[0]) return 0.f;
-        const float * X = fcol2(f, f.X, scheme, material);
+        const P X = fcol2(f, f.X, scheme, material);
         if (kinetic >= f.K[imax]) {
                 const float a = f.a_max[material];
                 const float b = f.b_max[scheme2(scheme) * f.n_materials +
@@ -131,7 +139,8 @@ PG_FN float fcel_grammage(const FloatTables & f, FloatCache & ch, int scheme,
 }
 
 /* Stopping power, as cel_energy_loss. */
-PG_FN float fcel_energy_loss(const FloatTables & f, FloatCache & ch,
+template <class P>
+PG_FN float fcel_energy_loss(const FloatTablesT<P> & f, FloatCache & ch,
     int scheme, int material, float kinetic)
 {
         const int imax = f.n_energies - 1;
@@ -145,12 +154,13 @@ PG_FN float fcel_energy_loss(const FloatTables & f, FloatCache & ch,
 }
 
 /* Straggling, as cel_straggling. */
+template <class P>
 PG_FN float fcel_straggling(
-    const FloatTables & f, FloatCache & ch, int material, float kinetic)
+    const FloatTablesT<P> & f, FloatCache & ch, int material, float kinetic)
 {
         const int imax = f.n_energies - 1;
         if (kinetic < f.K[0]) return 0.f;
-        const float * Om = fcol1(f, f.Omega, material);
+        const P Om = fcol1(f, f.Omega, material);
         if (kinetic >= f.K[imax]) {
                 const float r = (kinetic + f.mass) / (f.K[imax] + f.mass);
                 return Om[imax] * r * r;
@@ -160,25 +170,27 @@ PG_FN float fcel_straggling(
 }
 
 /* DEL cross section, as del_cross_section. */
+template <class P>
 PG_FN float fdel_cross_section(
-    const FloatTables & f, FloatCache & ch, int material, float kinetic)
+    const FloatTablesT<P> & f, FloatCache & ch, int material, float kinetic)
 {
         const int imax = f.n_energies - 1;
         if (kinetic < f.K[0]) return 0.f;
-        const float * CS = fcol1(f, f.CS, material);
+        const P CS = fcol1(f, f.CS, material);
         if (kinetic >= f.K[imax]) return CS[imax];
         return finterpolate(
             f, ch.k, f.K, CS, fcol1(f, f.CS_dK, material), kinetic);
 }
 
 /* Multiple scattering parameters, as table_get_msc. */
-PG_FN void ftable_get_msc(const FloatTables & f, FloatCache & ch, int scheme,
+template <class P>
+PG_FN void ftable_get_msc(const FloatTablesT<P> & f, FloatCache & ch, int scheme,
     int material, float kinetic, float * mu0, float * invlb1)
 {
         const int imax = f.n_energies - 1;
         const int s = scheme2(scheme) + 1;
-        const float * Mu0 = fcol1(f, f.Mu0, material);
-        const float * Ms1 = f.Ms1 + (s * f.n_materials + material) *
+        const P Mu0 = fcol1(f, f.Mu0, material);
+        const P Ms1 = f.Ms1 + (s * f.n_materials + material) *
             f.n_energies;
         if (kinetic < f.K[1]) {
                 *mu0 = Mu0[1];
@@ -187,8 +199,8 @@ PG_FN void ftable_get_msc(const FloatTables & f, FloatCache & ch, int scheme,
                 *mu0 = Mu0[imax];
                 *invlb1 = Ms1[imax] * f.K[imax] / kinetic;
         } else {
-                const float * m0 = fcol1(f, f.Mu0_dK, material);
-                const float * m1 = f.Ms1_dK +
+                const P m0 = fcol1(f, f.Mu0_dK, material);
+                const P m1 = f.Ms1_dK +
                     (s * f.n_materials + material) * f.n_energies;
                 int i = ch.k[0];
                 float x0 = f.K[i], x1 = f.K[i + 1];
@@ -215,12 +227,13 @@ PG_FN void ftable_get_msc(const FloatTables & f, FloatCache & ch, int scheme,
 }
 
 /* Hard elastic path, as coulomb_ehs_length. */
+template <class P>
 PG_FN float fcoulomb_ehs_length(
-    const FloatTables & f, FloatCache & ch, int material, float kinetic)
+    const FloatTablesT<P> & f, FloatCache & ch, int material, float kinetic)
 {
         const float p2 = kinetic * (kinetic + 2.f * f.mass);
         const int imax = f.n_energies - 1;
-        const float * Lb = fcol1(f, f.Lb, material);
+        const P Lb = fcol1(f, f.Lb, material);
         if (kinetic < f.K[1]) return Lb[1] / p2;
         if (kinetic >= f.K[imax]) return Lb[imax] / p2;
         int i = ch.k[0];

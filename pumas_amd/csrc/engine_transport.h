@@ -92,12 +92,12 @@ struct Recorder {
  * direction), the range to energy inversion, the event weights and the
  * geometry stay in double precision.
  */
-template <bool F32> struct EngineT {
+template <bool F32, class FT = FloatTables> struct EngineT {
         const PhysicsView & p;
         const ModelView & mv;
         const Config & c;
         const pg_geometry & g;
-        const FloatTables & ft;
+        const FT & ft;
         Recorder rec;
 
         /* Records the state of `w` in `medium` with `event`, if the history

@@ -18,6 +18,25 @@
 
 using namespace pg;
 
+/* Dynamic LDS of the FP32-bulk stepping kernel: a copy of the single
+ * precision tables, made by each block at its start. */
+extern __shared__ float pg_lds_floats[];
+
+/* A table column in that copy, as an offset in floats. Reads compile to LDS
+ * loads (ds_read) addressed from the symbol, so the table view holding these
+ * offsets stays in the kernel arguments (scalar registers). */
+struct LdsFloats {
+        uint32_t off;
+
+        PG_FN float operator[](int i) const { return pg_lds_floats[off + i]; }
+        PG_FN LdsFloats operator+(int k) const
+        {
+                return LdsFloats{ off + (uint32_t)k };
+        }
+};
+
+using LdsTables = FloatTablesT<LdsFloats>;
+
 /* ------------------------------------------------------------------------
  * Error reporting.
  * ------------------------------------------------------------------------ */
@@ -68,6 +87,8 @@ struct pumas_gpu_physics {
         PhysicsView view;
         ModelView models;
         FloatTables floats;     /* FP32 mirrors, in the same allocation */
+        LdsTables lds_floats;   /* the same, as offsets in the LDS copy */
+        uint32_t n_floats;      /* floats in the mirrors (multiple of 64) */
         int particle;
 };
 
@@ -287,6 +308,21 @@ extern "C" enum pumas_return pumas_gpu_physics_upload_memory(
                 ft.n_energies = v.n_energies;
                 ft.n_materials = v.n_materials;
                 ft.mass = (float)v.mass;
+
+                LdsTables & lt = ph->lds_floats;
+                LdsFloats * lfields[] = { &lt.K, &lt.X, &lt.X_dK, &lt.dE,
+                        &lt.dE_dK, &lt.Omega, &lt.Omega_dK, &lt.CS,
+                        &lt.CS_dK, &lt.Mu0, &lt.Mu0_dK, &lt.Ms1, &lt.Ms1_dK,
+                        &lt.Lb, &lt.a_max, &lt.b_max };
+                static_assert(sizeof(lfields) / sizeof(lfields[0]) ==
+                        sizeof(fields) / sizeof(fields[0]),
+                    "one LDS column per mirrored table");
+                for (int k = 0; k < n_mirrored; k++)
+                        lfields[k]->off = (uint32_t)float_at[k];
+                lt.n_energies = ft.n_energies;
+                lt.n_materials = ft.n_materials;
+                lt.mass = ft.mass;
+                ph->n_floats = (uint32_t)floats.size();
         }
 
         ModelView & mv = ph->models;
@@ -548,6 +584,8 @@ struct KernelArgs {
         PhysicsView p;
         ModelView mv;
         FloatTables ft;
+        LdsTables ftl;          /* ft in the stepping kernel's LDS copy */
+        uint32_t n_floats;      /* floats of that copy (from ft.K on) */
         Config c;
         pg_geometry g;
         struct pumas_state * states;
@@ -711,15 +749,33 @@ __global__ void k_round(unsigned long long * c, int parity)
  * batches. The out-of-line DEL call costs registers on every step, so large
  * batches use the variant without it.
  */
-template <bool F32, bool TAIL_DEL>
+template <bool F32, bool TAIL_DEL, bool LDS>
 __global__ void __launch_bounds__(256, 2) k_step(KernelArgs args, int parity)
 {
         KERNEL_ARGS(args);
-        const EngineT<F32> eng = { a.p, a.mv, a.c, a.g, a.ft, a.rec };
+        using Tables =
+            typename std::conditional<LDS, LdsTables, FloatTables>::type;
+        const Tables * tables;
+        if constexpr (LDS)
+                tables = &a.ftl;
+        else
+                tables = &a.ft;
+        const EngineT<F32, Tables> eng = { a.p, a.mv, a.c, a.g, *tables,
+                a.rec };
         const uint64_t below = lanes_below();
         const uint32_t * in = a.q_step[parity];
         const unsigned long long n_in = a.counters[C_STEP_N0 + parity];
         if (n_in == 0) return;
+        if constexpr (LDS) {
+                /* Copy of the single precision tables (tens of KB, read
+                 * from L2 by every block of the launch). */
+                const float4 * src = (const float4 *)a.ft.K;
+                float4 * dst = (float4 *)pg_lds_floats;
+                for (uint32_t i = threadIdx.x; i < a.n_floats / 4;
+                     i += blockDim.x)
+                        dst[i] = src[i];
+                __syncthreads();
+        }
         Walker w;
         w.phase = PHASE_EMPTY;
         uint32_t slot = 0;
@@ -860,10 +916,26 @@ struct Launch {
         int n_cu, block, per_cu, rounds_per_check;
         bool graph;
         int tail_del; /* <0 never, >0 always, 0: below TAIL_DEL_MAX */
+        unsigned lds_bytes; /* LDS copy of the FP32 tables, or 0 */
         unsigned step_grid, del_grid;
         hipStream_t stream;
         hipEvent_t * marks; /* [2 * rounds_per_check] or NULL */
 };
+
+template <bool F32, bool TAIL_DEL>
+void launch_step(const KernelArgs & a, int parity, const Launch & L)
+{
+        if constexpr (F32) {
+                if (L.lds_bytes) {
+                        hipLaunchKernelGGL((k_step<true, TAIL_DEL, true>),
+                            dim3(L.step_grid), dim3(L.block), L.lds_bytes,
+                            L.stream, a, parity);
+                        return;
+                }
+        }
+        hipLaunchKernelGGL((k_step<F32, TAIL_DEL, false>), dim3(L.step_grid),
+            dim3(L.block), 0, L.stream, a, parity);
+}
 
 template <bool F32>
 enum pumas_return launch_round(
@@ -874,15 +946,39 @@ enum pumas_return launch_round(
             parity);
         if (L.marks) HIP_TRY(hipEventRecord(L.marks[2 * r], stream));
         if (L.tail_del)
-                hipLaunchKernelGGL((k_step<F32, true>), dim3(L.step_grid),
-                    dim3(L.block), 0, stream, a, parity);
+                launch_step<F32, true>(a, parity, L);
         else
-                hipLaunchKernelGGL((k_step<F32, false>), dim3(L.step_grid),
-                    dim3(L.block), 0, stream, a, parity);
+                launch_step<F32, false>(a, parity, L);
         if (L.marks) HIP_TRY(hipEventRecord(L.marks[2 * r + 1], stream));
         hipLaunchKernelGGL(k_del<F32>, dim3(L.del_grid), dim3(256), 0,
             stream, a, parity);
         HIP_TRY(hipGetLastError());
+        return PUMAS_RETURN_SUCCESS;
+}
+
+/* Bytes of the LDS copy of the FP32 tables in the stepping kernel, or 0
+ * when `per_cu` copies do not fit in the LDS of a CU. */
+enum pumas_return lds_tables_size(const pumas_gpu_physics * physics,
+    int device, int per_cu, unsigned * bytes)
+{
+        *bytes = 0;
+        const size_t need = (size_t)physics->n_floats * sizeof(float);
+        int per_block = 0, per_sm = 0;
+        HIP_TRY(hipDeviceGetAttribute(
+            &per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+        HIP_TRY(hipDeviceGetAttribute(&per_sm,
+            hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device));
+        if ((need > (size_t)per_block) || (need * per_cu > (size_t)per_sm))
+                return PUMAS_RETURN_SUCCESS;
+        if (need > 65536) {
+                HIP_TRY(hipFuncSetAttribute(
+                    (const void *)k_step<true, true, true>,
+                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+                HIP_TRY(hipFuncSetAttribute(
+                    (const void *)k_step<true, false, true>,
+                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+        }
+        *bytes = (unsigned)need;
         return PUMAS_RETURN_SUCCESS;
 }
 
@@ -1065,6 +1161,8 @@ extern "C" enum pumas_return pumas_gpu_transport(
         a.p = physics->view;
         a.mv = physics->models;
         a.ft = physics->floats;
+        a.ftl = physics->lds_floats;
+        a.n_floats = physics->n_floats;
         a.c.energy_loss = context->mode.energy_loss;
         a.c.decay = context->mode.decay;
         a.c.direction = context->mode.direction;
@@ -1134,6 +1232,11 @@ extern "C" enum pumas_return pumas_gpu_transport(
                 HIP_TRY(hipEventRecord(e0, stream));
         }
         const bool f32 = (precision == PUMAS_GPU_PRECISION_FP32_BULK);
+        unsigned lds_bytes = 0;
+        if (f32 && !(options && options->lds_tables < 0)) {
+                rc = lds_tables_size(physics, device, per_cu, &lds_bytes);
+                if (rc != PUMAS_RETURN_SUCCESS) return rc;
+        }
         unsigned long long total_steps = 0, error = 0;
         /* The legacy NULL stream cannot be captured. */
         const bool graph = !(options && options->eager) && (stream != nullptr);
@@ -1149,7 +1252,7 @@ extern "C" enum pumas_return pumas_gpu_transport(
          * events recorded inside a graph cannot be timed. */
         const int tail_del = options ? options->tail_del : 0;
         const Launch L = { n_cu, block, per_cu, rounds_per_check, graph,
-                tail_del, 0, 0, stream,
+                tail_del, lds_bytes, 0, 0, stream,
                 (stats && !graph) ? marks.data() : nullptr };
         for (size_t base = 0; base < n; base += MAX_CHUNK) {
                 const size_t m = (n - base < MAX_CHUNK) ? n - base : MAX_CHUNK;

@@ -83,3 +83,22 @@ def test_fp32_forward_slabs(gpu_standard):
     steps = r32.total_steps / r64.total_steps - 1.0
     print(f"steps fp32 / fp64 - 1 = {steps:.2e}")
     assert abs(steps) < 1e-2
+
+
+@pytest.mark.parametrize("physics,tail_del", [("gpu_standard", 1),
+                                              ("gpu_standard", -1),
+                                              ("gpu_iron", 1),
+                                              ("gpu_iron", -1)])
+def test_fp32_lds_tables_invariance(request, physics, tail_del):
+    """The FP32 stepping kernel reads its tables from a copy in LDS or from
+    HBM: the same values, so the same results, bit for bit."""
+    phys = request.getfixturevalue(physics)
+    s = W.slabs_forward_states(4000, seed=23)
+    geom = pa.Geometry(phys, W.slabs_geometry())
+    ctx = W.slabs_forward_context()
+    kw = dict(seed=23, precision=pa.Precision.FP32_BULK, tail_del=tail_del)
+    hbm = pa.transport(phys, ctx, geom, s.copy(), lds_tables=-1, **kw)
+    lds = pa.transport(phys, ctx, geom, s.copy(), lds_tables=1, **kw)
+    assert (lds.states == hbm.states).all()
+    for k in ("events", "media", "steps", "draws", "rc"):
+        np.testing.assert_array_equal(getattr(lds, k), getattr(hbm, k))
