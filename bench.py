@@ -320,7 +320,7 @@ def main():
     traffic = None
     tpath = os.path.join(HERE, "profiles", "k_step_slabs_traffic.json")
     if (args.workload == "slabs" and os.path.exists(tpath) and
-            n == 10_000_000):
+            n == 10_000_000 and args.precision == "fp64"):
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
 

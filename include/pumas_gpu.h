@@ -246,9 +246,10 @@ enum pumas_gpu_precision {
 
 struct pumas_gpu_options {
         enum pumas_gpu_precision precision;
-        /* Threads per block (multiple of 64; 0 = default). */
+        /* Threads per block (multiple of 64, at most 512; 0 = 512). */
         int block_size;
-        /* Resident blocks per CU for the persistent kernel (0 = default). */
+        /* Resident blocks per CU for the persistent kernel (0 = 512 /
+         * block_size: 8 wavefronts per CU). */
         int blocks_per_cu;
         /* Safety cap on Monte Carlo steps per history (0 = 10^7). */
         int max_steps;
@@ -268,11 +269,12 @@ struct pumas_gpu_options {
          * < 0 never, 0 for batches below 2^22 histories (latency of small
          * requests). */
         int tail_del;
-        /* Single precision tables of the FP32-bulk stepping kernel read
-         * from a copy in each block's LDS when they fit beside the other
-         * resident blocks of a CU: 0 or > 0 when they fit, < 0 never (the
-         * tables are then read from HBM through the caches). Results do not
-         * depend on it. */
+        /* Tables of the per-step lookups (the FP32 mirrors in FP32-bulk,
+         * the FP64 step tables otherwise) read by the stepping kernel from
+         * a copy in each block's LDS when `blocks_per_cu` copies fit in the
+         * LDS of a CU: 0 or > 0 when they fit, < 0 never (the tables are
+         * then read from HBM through the caches). Results do not depend on
+         * it. */
         int lds_tables;
 };
 
@@ -309,6 +311,8 @@ struct pumas_gpu_stats {
                                    launches only; 0 with graph replays) */
         uint64_t step_launches; /* launches of the stepping kernel */
         uint64_t rounds;        /* host checks of the queue lengths */
+        uint64_t lds_bytes;     /* LDS copy of the step tables per block of
+                                   the stepping kernel (0: read from HBM) */
 };
 
 /*

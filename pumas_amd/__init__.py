@@ -271,6 +271,7 @@ class TransportResult:
     rc: object = None
     total_steps: int = 0
     kernel_ms: float = 0.0
+    lds_bytes: int = 0
 
 
 @dataclasses.dataclass
@@ -282,12 +283,13 @@ class TransportStats:
     step_kernel_ms: float = 0.0
     step_launches: int = 0
     rounds: int = 0
+    lds_bytes: int = 0
 
     @classmethod
     def from_c(cls, st):
         return cls(int(st.steps), int(st.histories), float(st.kernel_ms),
                    float(st.step_kernel_ms), int(st.step_launches),
-                   int(st.rounds))
+                   int(st.rounds), int(st.lds_bytes))
 
 
 def _options(precision, block_size, blocks_per_cu, max_steps,
@@ -351,6 +353,7 @@ def transport(physics, context, geometry, states, seed=0, first_history=0,
         _check(rc)
     res.total_steps = int(stats.steps)
     res.kernel_ms = float(stats.kernel_ms)
+    res.lds_bytes = int(stats.lds_bytes)
     return res
 
 

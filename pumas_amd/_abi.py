@@ -156,7 +156,7 @@ class GpuStats(ctypes.Structure):
                 ("kernel_ms", ctypes.c_double),
                 ("step_kernel_ms", ctypes.c_double),
                 ("step_launches", ctypes.c_uint64),
-                ("rounds", ctypes.c_uint64)]
+                ("rounds", ctypes.c_uint64), ("lds_bytes", ctypes.c_uint64)]
 
 
 # Every entry point declared by include/pumas_gpu.h.

@@ -23,8 +23,10 @@ struct Shape {
 /* Launch shapes at the stepping kernel's full residency (8 wavefronts per
  * CU, the limit of its 256 registers per lane). Fewer resident wavefronts
  * only look faster on samples so small that the tail of the launch (the
- * longest histories) dominates. */
-const Shape SHAPES[] = { { 256, 2 }, { 128, 4 }, { 64, 8 } };
+ * longest histories) dominates. One block per CU holds one LDS copy of the
+ * step tables, which the FP64 ones need; two blocks hold two copies of the
+ * FP32 ones; smaller blocks read the tables from HBM. */
+const Shape SHAPES[] = { { 512, 1 }, { 256, 2 }, { 128, 4 }, { 64, 8 } };
 const int EHS_BATCHES[] = { 8, 16, 24, 32 };
 
 } /* namespace */

@@ -224,34 +224,35 @@ struct Config {
 /* Table accessors, src/pumas.c:4260-4752. */
 PG_FN int scheme2(int scheme) { return (scheme > MODE_MIXED) ? MODE_MIXED : scheme; }
 
-PG_FN const double * tab2(const PhysicsView & p, const double * t, int scheme,
-    int material, int row)
+template <class V, class C>
+PG_FN C tab2(const V & p, C t, int scheme, int material, int row)
 {
         return t + (scheme2(scheme) * p.n_materials + material) * p.n_energies +
             row;
 }
 
-PG_FN const double * tab1(
-    const PhysicsView & p, const double * t, int material, int row)
+template <class V, class C>
+PG_FN C tab1(const V & p, C t, int material, int row)
 {
         return t + material * p.n_energies + row;
 }
 
-PG_FN const double * tab_NI_el(const PhysicsView & p, const double * t,
-    int scheme, int material, int row)
+template <class V, class C>
+PG_FN C tab_NI_el(const V & p, C t, int scheme, int material, int row)
 {
         const int s = (scheme >= MODE_MIXED) ? MODE_MIXED : MODE_CSDA;
         return t + (s * p.n_materials + material) * p.n_energies + row;
 }
 
-PG_FN const double * tab_Ms1(const PhysicsView & p, const double * t,
-    int scheme, int material, int row)
+template <class V, class C>
+PG_FN C tab_Ms1(const V & p, C t, int scheme, int material, int row)
 {
         return t + ((scheme2(scheme) + 1) * p.n_materials + material) *
             p.n_energies + row;
 }
 
-PG_FN double b_max(const PhysicsView & p, int scheme, int material)
+template <class V>
+PG_FN double b_max(const V & p, int scheme, int material)
 {
         return p.b_max[scheme2(scheme) * p.n_materials + material];
 }
@@ -269,7 +270,8 @@ PG_FN double pchip(double t, double p0, double p1, double m0, double m1)
  * n - 1 above it: the result of table_index / table_bracket
  * (src/pumas.c:4174-4233).
  */
-PG_FN int table_index(const double * table, int n, double value)
+template <class C>
+PG_FN int table_index(C table, int n, double value)
 {
         const int imax = n - 1;
         if (value < table[0]) return -1;
@@ -308,8 +310,8 @@ struct Cache {
  * (one round trip to memory): between two lookups of a history the value
  * usually moves by less than a row. The bisection is the fallback.
  */
-PG_FN int cached_index_miss(
-    const double * table, int n, double value, int * last)
+template <class C>
+PG_FN int cached_index_miss(C table, int n, double value, int * last)
 {
         const int i0 = last[0];
         const double below = table[(i0 > 0) ? i0 - 1 : 0];
@@ -334,7 +336,8 @@ PG_FN int cached_index_miss(
         return i1;
 }
 
-PG_FN int cached_index(const double * table, int n, double value, int * last)
+template <class C>
+PG_FN int cached_index(C table, int n, double value, int * last)
 {
         if ((value >= table[last[0]]) && (value < table[last[0] + 1]))
                 return last[0];
@@ -342,7 +345,8 @@ PG_FN int cached_index(const double * table, int n, double value, int * last)
 }
 
 /* Bracketing row in the kinetic energy grid. */
-PG_FN int index_K(const PhysicsView & p, Cache & ch, double kinetic)
+template <class V>
+PG_FN int index_K(const V & p, Cache & ch, double kinetic)
 {
         return cached_index(p.K, p.n_energies, kinetic, ch.k);
 }
@@ -354,8 +358,8 @@ PG_FN int index_K(const PhysicsView & p, Cache & ch, double kinetic)
  * kinetic energy changes little between lookups) costs one round trip to
  * memory instead of two. A miss reloads them.
  */
-PG_FN double interpolate(const PhysicsView & p, int * last, const double * tx,
-    const double * ty, const double * tm, double x)
+template <class V, class A, class B, class C>
+PG_FN double interpolate(const V & p, int * last, A tx, B ty, C tm, double x)
 {
         int i1 = last[0];
         double x0 = tx[i1], x1 = tx[i1 + 1];
@@ -378,8 +382,8 @@ PG_FN double interpolate(const PhysicsView & p, int * last, const double * tx,
 }
 
 /* cel_grammage, src/pumas.c:3644-3669. */
-PG_FN double cel_grammage(
-    const PhysicsView & p, Cache & ch, int scheme, int material, double kinetic)
+template <class V>
+PG_FN double cel_grammage(const V & p, Cache & ch, int scheme, int material, double kinetic)
 {
         const int imax = p.n_energies - 1;
         if (kinetic < p.K[0]) return 0.;
@@ -396,11 +400,11 @@ PG_FN double cel_grammage(
 }
 
 /* cel_grammage_as_time, src/pumas.c:3680-3709. */
-PG_FN double cel_grammage_as_time(
-    const PhysicsView & p, Cache & ch, int scheme, int material, double time)
+template <class V>
+PG_FN double cel_grammage_as_time(const V & p, Cache & ch, int scheme, int material, double time)
 {
         const int imax = p.n_energies - 1;
-        const double * T = tab2(p, p.T, scheme, material, 0);
+        const auto T = tab2(p, p.T, scheme, material, 0);
         if (time < T[0]) return 0.;
         if (time >= T[imax]) {
                 const double a = p.a_max[material];
@@ -419,8 +423,8 @@ PG_FN double cel_grammage_as_time(
 }
 
 /* cel_proper_time, src/pumas.c:3720-3745. */
-PG_FN double cel_proper_time(
-    const PhysicsView & p, Cache & ch, int scheme, int material, double kinetic)
+template <class V>
+PG_FN double cel_proper_time(const V & p, Cache & ch, int scheme, int material, double kinetic)
 {
         const int imax = p.n_energies - 1;
         if (kinetic < p.K[0]) return 0.;
@@ -437,11 +441,11 @@ PG_FN double cel_proper_time(
 }
 
 /* cel_kinetic_energy, src/pumas.c:3758-3784. */
-PG_FN double cel_kinetic_energy(
-    const PhysicsView & p, Cache & ch, int scheme, int material, double grammage)
+template <class V>
+PG_FN double cel_kinetic_energy(const V & p, Cache & ch, int scheme, int material, double grammage)
 {
         const int imax = p.n_energies - 1;
-        const double * X = tab2(p, p.X, scheme, material, 0);
+        const auto X = tab2(p, p.X, scheme, material, 0);
         if (grammage < X[0]) return 0.;
         if (grammage >= X[imax]) {
                 const double a = p.a_max[material];
@@ -455,8 +459,8 @@ PG_FN double cel_kinetic_energy(
 }
 
 /* cel_energy_loss, src/pumas.c:3796-3814. */
-PG_FN double cel_energy_loss(
-    const PhysicsView & p, Cache & ch, int scheme, int material, double kinetic)
+template <class V>
+PG_FN double cel_energy_loss(const V & p, Cache & ch, int scheme, int material, double kinetic)
 {
         const int imax = p.n_energies - 1;
         if (kinetic < p.K[0]) return 0.;
@@ -469,7 +473,8 @@ PG_FN double cel_energy_loss(
 }
 
 /* cel_straggling, src/pumas.c:3826-3846. */
-PG_FN double cel_straggling(const PhysicsView & p, Cache & ch, int material, double kinetic)
+template <class V>
+PG_FN double cel_straggling(const V & p, Cache & ch, int material, double kinetic)
 {
         const int imax = p.n_energies - 1;
         if (kinetic < p.K[0]) return 0.;
@@ -483,8 +488,8 @@ PG_FN double cel_straggling(const PhysicsView & p, Cache & ch, int material, dou
 }
 
 /* del_cross_section, src/pumas.c:3895-3912. */
-PG_FN double del_cross_section(
-    const PhysicsView & p, Cache & ch, int material, double kinetic)
+template <class V>
+PG_FN double del_cross_section(const V & p, Cache & ch, int material, double kinetic)
 {
         const int imax = p.n_energies - 1;
         if (kinetic < p.K[0]) return 0.;
@@ -494,8 +499,8 @@ PG_FN double del_cross_section(
 }
 
 /* del_interaction_length, src/pumas.c:3923-3947. */
-PG_FN double del_interaction_length(
-    const PhysicsView & p, Cache & ch, int material, double kinetic)
+template <class V>
+PG_FN double del_interaction_length(const V & p, Cache & ch, int material, double kinetic)
 {
         const int imax = p.n_energies - 1;
         if (kinetic < p.K[0]) return 0.;
@@ -514,11 +519,11 @@ PG_FN double del_interaction_length(
 }
 
 /* del_kinetic_from_interaction_length, src/pumas.c:3959-3987. */
-PG_FN double del_kinetic_from_interaction_length(
-    const PhysicsView & p, Cache & ch, int material, double nI)
+template <class V>
+PG_FN double del_kinetic_from_interaction_length(const V & p, Cache & ch, int material, double nI)
 {
         const int imax = p.n_energies - 1;
-        const double * NI = tab1(p, p.NI_in, material, 0);
+        const auto NI = tab1(p, p.NI_in, material, 0);
         if (nI < NI[0]) return 0.;
         if (nI >= NI[imax]) {
                 const double k0 = p.K[imax];
@@ -534,11 +539,11 @@ PG_FN double del_kinetic_from_interaction_length(
 }
 
 /* ehs_interaction_length, src/pumas.c:3998-4023. */
-PG_FN double ehs_interaction_length(
-    const PhysicsView & p, Cache & ch, int scheme, int material, double kinetic)
+template <class V>
+PG_FN double ehs_interaction_length(const V & p, Cache & ch, int scheme, int material, double kinetic)
 {
         const int imax = p.n_energies - 1;
-        const double * NI = tab_NI_el(p, p.NI_el, scheme, material, 0);
+        const auto NI = tab_NI_el(p, p.NI_el, scheme, material, 0);
         if (kinetic < p.K[0]) return 0.;
         if (kinetic >= p.K[imax]) {
                 const double k0 = p.K[imax - 1];
@@ -552,11 +557,11 @@ PG_FN double ehs_interaction_length(
 }
 
 /* ehs_kinetic_from_interaction_length, src/pumas.c:4034-4058. */
-PG_FN double ehs_kinetic_from_interaction_length(
-    const PhysicsView & p, Cache & ch, int scheme, int material, double nI)
+template <class V>
+PG_FN double ehs_kinetic_from_interaction_length(const V & p, Cache & ch, int scheme, int material, double nI)
 {
         const int imax = p.n_energies - 1;
-        const double * NI = tab_NI_el(p, p.NI_el, scheme, material, 0);
+        const auto NI = tab_NI_el(p, p.NI_el, scheme, material, 0);
         if (nI < NI[0]) return 0.;
         if (nI >= NI[imax]) {
                 const double k0 = p.K[imax - 1];
@@ -571,7 +576,8 @@ PG_FN double ehs_kinetic_from_interaction_length(
 }
 
 /* table_get_msc, src/pumas.c:4077-4111. */
-PG_FN void table_get_msc(const PhysicsView & p, Cache & ch, int scheme, int material,
+template <class V>
+PG_FN void table_get_msc(const V & p, Cache & ch, int scheme, int material,
     double kinetic, double * mu0, double * invlb1)
 {
         const int imax = p.n_energies - 1;
@@ -586,10 +592,10 @@ PG_FN void table_get_msc(const PhysicsView & p, Cache & ch, int scheme, int mate
         } else {
                 /* Speculative rows at the first cached bracket, as in
                  * interpolate. */
-                const double * y0 = tab1(p, p.Mu0, material, 0);
-                const double * m0 = tab1(p, p.Mu0_dK, material, 0);
-                const double * y1 = tab_Ms1(p, p.Ms1, scheme, material, 0);
-                const double * m1 = tab_Ms1(p, p.Ms1_dK, scheme, material, 0);
+                const auto y0 = tab1(p, p.Mu0, material, 0);
+                const auto m0 = tab1(p, p.Mu0_dK, material, 0);
+                const auto y1 = tab_Ms1(p, p.Ms1, scheme, material, 0);
+                const auto m1 = tab_Ms1(p, p.Ms1_dK, scheme, material, 0);
                 int i = ch.k[0];
                 double x0 = p.K[i], x1 = p.K[i + 1];
                 double a0 = y0[i], a1 = y0[i + 1], b0 = m0[i], b1 = m0[i + 1];
@@ -615,12 +621,12 @@ PG_FN void table_get_msc(const PhysicsView & p, Cache & ch, int scheme, int mate
 }
 
 /* coulomb_ehs_length, src/pumas.c:7446-7466. */
-PG_FN double coulomb_ehs_length(
-    const PhysicsView & p, Cache & ch, int material, double kinetic)
+template <class V>
+PG_FN double coulomb_ehs_length(const V & p, Cache & ch, int material, double kinetic)
 {
         const double p2 = kinetic * (kinetic + 2. * p.mass);
         const int imax = p.n_energies - 1;
-        const double * Lb = tab1(p, p.Lb, material, 0);
+        const auto Lb = tab1(p, p.Lb, material, 0);
         if (kinetic < p.K[1]) {
                 return Lb[1] / p2;
         } else if (kinetic >= p.K[imax]) {

@@ -92,12 +92,16 @@ struct Recorder {
  * direction), the range to energy inversion, the event weights and the
  * geometry stay in double precision.
  */
-template <bool F32, class FT = FloatTables> struct EngineT {
+template <bool F32, class FT = FloatTables, class ST = PhysicsView>
+struct EngineT {
         const PhysicsView & p;
         const ModelView & mv;
         const Config & c;
         const pg_geometry & g;
         const FT & ft;
+        /* The FP64 tables read by every step (the same tables as `p`, or a
+         * copy of them in LDS). */
+        const ST & st;
         Recorder rec;
 
         /* Records the state of `w` in `medium` with `event`, if the history
@@ -147,7 +151,7 @@ template <bool F32, class FT = FloatTables> struct EngineT {
                         return fcel_grammage(
                             ft, t.fx, scheme, material, (float)kinetic);
                 else
-                        return cel_grammage(p, t.ix, scheme, material, kinetic);
+                        return cel_grammage(st, t.ix, scheme, material, kinetic);
         }
 
         PG_FN double bulk_energy_loss(Track & t, int scheme, int material,
@@ -158,7 +162,7 @@ template <bool F32, class FT = FloatTables> struct EngineT {
                             ft, t.fx, scheme, material, (float)kinetic);
                 else
                         return cel_energy_loss(
-                            p, t.ix, scheme, material, kinetic);
+                            st, t.ix, scheme, material, kinetic);
         }
 
         PG_FN double bulk_straggling(Track & t, int material,
@@ -168,7 +172,7 @@ template <bool F32, class FT = FloatTables> struct EngineT {
                         return fcel_straggling(
                             ft, t.fx, material, (float)kinetic);
                 else
-                        return cel_straggling(p, t.ix, material, kinetic);
+                        return cel_straggling(st, t.ix, material, kinetic);
         }
 
         PG_FN double bulk_del_cross_section(Track & t, int material,
@@ -178,7 +182,7 @@ template <bool F32, class FT = FloatTables> struct EngineT {
                         return fdel_cross_section(
                             ft, t.fx, material, (float)kinetic);
                 else
-                        return del_cross_section(p, t.ix, material, kinetic);
+                        return del_cross_section(st, t.ix, material, kinetic);
         }
 
         PG_FN double bulk_ehs_length(Track & t, int material,
@@ -188,7 +192,7 @@ template <bool F32, class FT = FloatTables> struct EngineT {
                         return fcoulomb_ehs_length(
                             ft, t.fx, material, (float)kinetic);
                 else
-                        return coulomb_ehs_length(p, t.ix, material, kinetic);
+                        return coulomb_ehs_length(st, t.ix, material, kinetic);
         }
 
         PG_FN void bulk_msc(Track & t, int scheme, int material,
@@ -202,7 +206,7 @@ template <bool F32, class FT = FloatTables> struct EngineT {
                         *invlb1 = l;
                 } else {
                         table_get_msc(
-                            p, t.ix, scheme, material, kinetic, mu0, invlb1);
+                            st, t.ix, scheme, material, kinetic, mu0, invlb1);
                 }
         }
 
@@ -559,7 +563,7 @@ template <bool F32, class FT = FloatTables> struct EngineT {
                 if (foreseen == EVENT_NONE) return;
 
                 const double X = Xi +
-                    s * (Xf - cel_grammage(p, t.ix, scheme, material, kinetic_limit));
+                    s * (Xf - cel_grammage(st, t.ix, scheme, material, kinetic_limit));
                 if ((*grammage_max <= 0) || (X < *grammage_max)) {
                         *grammage_max = X;
                         t.step_foreseen = foreseen;
@@ -573,7 +577,7 @@ template <bool F32, class FT = FloatTables> struct EngineT {
                 const int scheme = c.energy_loss;
                 const double s = sgn();
                 double k1, r = 0.;
-                k1 = cel_kinetic_energy(p, t.ix, scheme, material, Xtot - s * dX);
+                k1 = cel_kinetic_energy(st, t.ix, scheme, material, Xtot - s * dX);
                 const double dk0 = fabs(t.energy - k1);
                 if ((k1 > 0.) && (dk0 > 0.)) {
                         const double Omega0 =
@@ -804,7 +808,7 @@ template <bool F32, class FT = FloatTables> struct EngineT {
                                         k1 = 0.;
                                 }
                         } else {
-                                k1 = cel_kinetic_energy(p, t.ix, scheme, material, X);
+                                k1 = cel_kinetic_energy(st, t.ix, scheme, material, X);
                         }
                 } else if (scheme == MODE_STRAGGLED) {
                         double ratio;
@@ -853,7 +857,7 @@ template <bool F32, class FT = FloatTables> struct EngineT {
                                 const double X_del =
                                     bulk_neglog_uniform(t) / xs_del;
                                 if (X_del >= Xmax) break;
-                                const double k_h = cel_kinetic_energy(p, t.ix, scheme, material, Xtot - sg * X_del);
+                                const double k_h = cel_kinetic_energy(st, t.ix, scheme, material, Xtot - sg * X_del);
                                 const double k_del = t.energy -
                                     sg * ratio * fabs(t.energy - k_h);
                                 if (k_del <= 0.) break;
@@ -883,7 +887,7 @@ template <bool F32, class FT = FloatTables> struct EngineT {
                                 const double X_ehs =
                                     bulk_neglog_uniform(t) * lb_ehs;
                                 if (X_ehs >= Xmax) break;
-                                const double k_h = cel_kinetic_energy(p, t.ix, scheme, material, Xtot - sg * X_ehs);
+                                const double k_h = cel_kinetic_energy(st, t.ix, scheme, material, Xtot - sg * X_ehs);
                                 const double k_ehs = t.energy -
                                     sg * ratio * fabs(t.energy - k_h);
                                 if (k_ehs <= 0.) break;
@@ -923,7 +927,7 @@ template <bool F32, class FT = FloatTables> struct EngineT {
                         if (!(event & EVENT_LIMIT_ENERGY)) {
                                 if (scheme <= MODE_MIXED) {
                                         if (scheme != MODE_DISABLED)
-                                                k1 = cel_kinetic_energy(p, t.ix,
+                                                k1 = cel_kinetic_energy(st, t.ix,
                                                     scheme, material,
                                                     Xtot - sg * (t.grammage -
                                                                     Xi));
@@ -972,7 +976,7 @@ template <bool F32, class FT = FloatTables> struct EngineT {
                                                 t.grammage = Xi + dxT;
                                                 const double xf =
                                                     Xtot - sg * dxT;
-                                                k1 = cel_kinetic_energy(p, t.ix, scheme, material, xf);
+                                                k1 = cel_kinetic_energy(st, t.ix, scheme, material, xf);
                                         }
                                 }
                         }

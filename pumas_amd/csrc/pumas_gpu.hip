@@ -37,6 +37,30 @@ struct LdsFloats {
 
 using LdsTables = FloatTablesT<LdsFloats>;
 
+/* The same for the FP64 stepping kernel: the tables read by every step. */
+extern __shared__ double pg_lds_doubles[];
+
+struct LdsDoubles {
+        uint32_t off;
+
+        PG_FN double operator[](int i) const { return pg_lds_doubles[off + i]; }
+        PG_FN double operator*() const { return pg_lds_doubles[off]; }
+        PG_FN LdsDoubles operator+(int k) const
+        {
+                return LdsDoubles{ off + (uint32_t)k };
+        }
+};
+
+/* The members of PhysicsView that the per-step lookups read (CEL range,
+ * stopping power and its inverse, straggling, DEL cross section, MSC and
+ * EHS parameters), as columns of the LDS copy. */
+struct LdsStepTables {
+        int n_energies, n_materials;
+        double mass;
+        LdsDoubles K, X, X_dK, K_dX, dE, dE_dK, Omega, Omega_dK;
+        LdsDoubles CS, CS_dK, Mu0, Mu0_dK, Ms1, Ms1_dK, Lb, a_max, b_max;
+};
+
 /* ------------------------------------------------------------------------
  * Error reporting.
  * ------------------------------------------------------------------------ */
@@ -89,6 +113,9 @@ struct pumas_gpu_physics {
         FloatTables floats;     /* FP32 mirrors, in the same allocation */
         LdsTables lds_floats;   /* the same, as offsets in the LDS copy */
         uint32_t n_floats;      /* floats in the mirrors (multiple of 64) */
+        const double * step;    /* packed copy of the FP64 step tables */
+        LdsStepTables lds_step; /* their columns, as offsets in that copy */
+        uint32_t n_step;        /* doubles in that copy (multiple of 8) */
         int particle;
 };
 
@@ -194,11 +221,33 @@ extern "C" enum pumas_return pumas_gpu_physics_upload_memory(
                 while (floats.size() % 64) floats.push_back(0.f);
         }
 
+        /* The FP64 tables read by every step, packed for the LDS copy of
+         * the FP64 stepping kernel (columns padded to 64 B). */
+        static const char * const stepped[] = { "K", "X", "X_dK", "K_dX",
+                "dE", "dE_dK", "Omega", "Omega_dK", "CS", "CS_dK", "Mu0",
+                "Mu0_dK", "Ms1", "Ms1_dK", "Lb", "a_max", "b_max" };
+        const int n_stepped = sizeof(stepped) / sizeof(stepped[0]);
+        std::vector<double> step;
+        std::vector<size_t> step_at(n_stepped);
+        for (int k = 0; k < n_stepped; k++) {
+                const Section * sec = find(sections, stepped[k]);
+                if ((sec == nullptr) || (sec->dtype != PTAB_F64))
+                        return pg_fail(PUMAS_RETURN_FORMAT_ERROR,
+                            "PTAB section `%s' is missing", stepped[k]);
+                step_at[k] = step.size();
+                const double * src =
+                    reinterpret_cast<const double *>(image + sec->offset);
+                step.insert(step.end(), src, src + sec->count);
+                while (step.size() % 8) step.push_back(0.);
+        }
+
         /* Device image: the PTAB file as is, then the derived sections. */
         const size_t extra_offset = (size + 255) / 256 * 256;
         const size_t float_offset = (extra_offset +
             component0.size() * sizeof(int32_t) + 255) / 256 * 256;
-        const size_t total = float_offset + floats.size() * sizeof(float);
+        const size_t step_offset = (float_offset +
+            floats.size() * sizeof(float) + 255) / 256 * 256;
+        const size_t total = step_offset + step.size() * sizeof(double);
         void * device = nullptr;
         HIP_TRY(hipMalloc(&device, total));
         hipError_t e1 = hipMemcpy(device, image, size, hipMemcpyHostToDevice);
@@ -208,7 +257,10 @@ extern "C" enum pumas_return pumas_gpu_physics_upload_memory(
         hipError_t e3 = hipMemcpy((uint8_t *)device + float_offset,
             floats.data(), floats.size() * sizeof(float),
             hipMemcpyHostToDevice);
-        if ((e1 != hipSuccess) || (e2 != hipSuccess) || (e3 != hipSuccess)) {
+        hipError_t e4 = hipMemcpy((uint8_t *)device + step_offset,
+            step.data(), step.size() * sizeof(double), hipMemcpyHostToDevice);
+        if ((e1 != hipSuccess) || (e2 != hipSuccess) || (e3 != hipSuccess) ||
+            (e4 != hipSuccess)) {
                 (void)hipFree(device);
                 return pg_fail(PUMAS_RETURN_INTERNAL_ERROR,
                     "could not copy the tables to the device");
@@ -323,6 +375,24 @@ extern "C" enum pumas_return pumas_gpu_physics_upload_memory(
                 lt.n_materials = ft.n_materials;
                 lt.mass = ft.mass;
                 ph->n_floats = (uint32_t)floats.size();
+        }
+        {
+                LdsStepTables & st = ph->lds_step;
+                LdsDoubles * columns[] = { &st.K, &st.X, &st.X_dK, &st.K_dX,
+                        &st.dE, &st.dE_dK, &st.Omega, &st.Omega_dK, &st.CS,
+                        &st.CS_dK, &st.Mu0, &st.Mu0_dK, &st.Ms1, &st.Ms1_dK,
+                        &st.Lb, &st.a_max, &st.b_max };
+                static_assert(sizeof(columns) / sizeof(columns[0]) ==
+                        sizeof(stepped) / sizeof(stepped[0]),
+                    "one LDS column per stepped table");
+                for (int k = 0; k < n_stepped; k++)
+                        columns[k]->off = (uint32_t)step_at[k];
+                st.n_energies = v.n_energies;
+                st.n_materials = v.n_materials;
+                st.mass = v.mass;
+                ph->step = (const double *)((const uint8_t *)device +
+                    step_offset);
+                ph->n_step = (uint32_t)step.size();
         }
 
         ModelView & mv = ph->models;
@@ -586,6 +656,9 @@ struct KernelArgs {
         FloatTables ft;
         LdsTables ftl;          /* ft in the stepping kernel's LDS copy */
         uint32_t n_floats;      /* floats of that copy (from ft.K on) */
+        const double * step;    /* FP64 step tables, packed */
+        LdsStepTables stl;      /* their columns in the LDS copy */
+        uint32_t n_step;        /* doubles of that copy */
         Config c;
         pg_geometry g;
         struct pumas_state * states;
@@ -711,7 +784,7 @@ template <bool F32>
 __global__ void __launch_bounds__(256) k_init(KernelArgs args)
 {
         KERNEL_ARGS(args);
-        const EngineT<F32> eng = { a.p, a.mv, a.c, a.g, a.ft, a.rec };
+        const EngineT<F32> eng = { a.p, a.mv, a.c, a.g, a.ft, a.p, a.rec };
         const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
         const bool active = i < a.n;
         Walker w;
@@ -750,29 +823,40 @@ __global__ void k_round(unsigned long long * c, int parity)
  * batches use the variant without it.
  */
 template <bool F32, bool TAIL_DEL, bool LDS>
-__global__ void __launch_bounds__(256, 2) k_step(KernelArgs args, int parity)
+__global__ void __launch_bounds__(512, 2) k_step(KernelArgs args, int parity)
 {
         KERNEL_ARGS(args);
-        using Tables =
-            typename std::conditional<LDS, LdsTables, FloatTables>::type;
-        const Tables * tables;
-        if constexpr (LDS)
-                tables = &a.ftl;
+        /* With LDS, the tables of the step lookups in the working precision
+         * are read from a copy in LDS: the FP32 mirrors (FP32-bulk) or the
+         * packed FP64 step tables. */
+        using FT = typename std::conditional<LDS && F32, LdsTables,
+            FloatTables>::type;
+        using ST = typename std::conditional<LDS && !F32, LdsStepTables,
+            PhysicsView>::type;
+        const FT * ft;
+        const ST * st;
+        if constexpr (LDS && F32)
+                ft = &a.ftl;
         else
-                tables = &a.ft;
-        const EngineT<F32, Tables> eng = { a.p, a.mv, a.c, a.g, *tables,
+                ft = &a.ft;
+        if constexpr (LDS && !F32)
+                st = &a.stl;
+        else
+                st = &a.p;
+        const EngineT<F32, FT, ST> eng = { a.p, a.mv, a.c, a.g, *ft, *st,
                 a.rec };
         const uint64_t below = lanes_below();
         const uint32_t * in = a.q_step[parity];
         const unsigned long long n_in = a.counters[C_STEP_N0 + parity];
         if (n_in == 0) return;
         if constexpr (LDS) {
-                /* Copy of the single precision tables (tens of KB, read
-                 * from L2 by every block of the launch). */
-                const float4 * src = (const float4 *)a.ft.K;
+                /* The copy (tens of KB to 160 KB, read from L2 by every
+                 * block of the launch), in 16 B pieces. */
+                const float4 * src = F32 ? (const float4 *)a.ft.K :
+                                           (const float4 *)a.step;
+                const uint32_t n4 = F32 ? a.n_floats / 4 : a.n_step / 2;
                 float4 * dst = (float4 *)pg_lds_floats;
-                for (uint32_t i = threadIdx.x; i < a.n_floats / 4;
-                     i += blockDim.x)
+                for (uint32_t i = threadIdx.x; i < n4; i += blockDim.x)
                         dst[i] = src[i];
                 __syncthreads();
         }
@@ -852,7 +936,7 @@ template <bool F32>
 __global__ void __launch_bounds__(256, 2) k_del(KernelArgs args, int parity)
 {
         KERNEL_ARGS(args);
-        const EngineT<F32> eng = { a.p, a.mv, a.c, a.g, a.ft, a.rec };
+        const EngineT<F32> eng = { a.p, a.mv, a.c, a.g, a.ft, a.p, a.rec };
         const unsigned long long n = a.counters[C_DEL_N];
         const uint32_t * queue = a.q_del;
         const int lane = threadIdx.x & 63;
@@ -925,16 +1009,13 @@ struct Launch {
 template <bool F32, bool TAIL_DEL>
 void launch_step(const KernelArgs & a, int parity, const Launch & L)
 {
-        if constexpr (F32) {
-                if (L.lds_bytes) {
-                        hipLaunchKernelGGL((k_step<true, TAIL_DEL, true>),
-                            dim3(L.step_grid), dim3(L.block), L.lds_bytes,
-                            L.stream, a, parity);
-                        return;
-                }
-        }
-        hipLaunchKernelGGL((k_step<F32, TAIL_DEL, false>), dim3(L.step_grid),
-            dim3(L.block), 0, L.stream, a, parity);
+        if (L.lds_bytes)
+                hipLaunchKernelGGL((k_step<F32, TAIL_DEL, true>),
+                    dim3(L.step_grid), dim3(L.block), L.lds_bytes, L.stream,
+                    a, parity);
+        else
+                hipLaunchKernelGGL((k_step<F32, TAIL_DEL, false>),
+                    dim3(L.step_grid), dim3(L.block), 0, L.stream, a, parity);
 }
 
 template <bool F32>
@@ -956,13 +1037,25 @@ enum pumas_return launch_round(
         return PUMAS_RETURN_SUCCESS;
 }
 
-/* Bytes of the LDS copy of the FP32 tables in the stepping kernel, or 0
+template <bool F32>
+enum pumas_return allow_lds(int bytes)
+{
+        HIP_TRY(hipFuncSetAttribute((const void *)k_step<F32, true, true>,
+            hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        HIP_TRY(hipFuncSetAttribute((const void *)k_step<F32, false, true>,
+            hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        return PUMAS_RETURN_SUCCESS;
+}
+
+/* Bytes of the LDS copy of the step tables in the stepping kernel, or 0
  * when `per_cu` copies do not fit in the LDS of a CU. */
 enum pumas_return lds_tables_size(const pumas_gpu_physics * physics,
-    int device, int per_cu, unsigned * bytes)
+    int device, int per_cu, bool f32, unsigned * bytes)
 {
         *bytes = 0;
-        const size_t need = (size_t)physics->n_floats * sizeof(float);
+        const size_t need = f32 ?
+            (size_t)physics->n_floats * sizeof(float) :
+            (size_t)physics->n_step * sizeof(double);
         int per_block = 0, per_sm = 0;
         HIP_TRY(hipDeviceGetAttribute(
             &per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
@@ -971,12 +1064,10 @@ enum pumas_return lds_tables_size(const pumas_gpu_physics * physics,
         if ((need > (size_t)per_block) || (need * per_cu > (size_t)per_sm))
                 return PUMAS_RETURN_SUCCESS;
         if (need > 65536) {
-                HIP_TRY(hipFuncSetAttribute(
-                    (const void *)k_step<true, true, true>,
-                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
-                HIP_TRY(hipFuncSetAttribute(
-                    (const void *)k_step<true, false, true>,
-                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+                const enum pumas_return rc = f32 ?
+                    allow_lds<true>((int)need) :
+                    allow_lds<false>((int)need);
+                if (rc != PUMAS_RETURN_SUCCESS) return rc;
         }
         *bytes = (unsigned)need;
         return PUMAS_RETURN_SUCCESS;
@@ -1163,6 +1254,9 @@ extern "C" enum pumas_return pumas_gpu_transport(
         a.ft = physics->floats;
         a.ftl = physics->lds_floats;
         a.n_floats = physics->n_floats;
+        a.step = physics->step;
+        a.stl = physics->lds_step;
+        a.n_step = physics->n_step;
         a.c.energy_loss = context->mode.energy_loss;
         a.c.decay = context->mode.decay;
         a.c.direction = context->mode.direction;
@@ -1207,15 +1301,17 @@ extern "C" enum pumas_return pumas_gpu_transport(
         int n_cu = 0;
         HIP_TRY(hipDeviceGetAttribute(
             &n_cu, hipDeviceAttributeMultiprocessorCount, device));
+        /* Default launch: 8 resident wavefronts per CU, as one block of
+         * 512 lanes (one LDS copy of the step tables per CU). */
         const int block = (options && options->block_size > 0) ?
             options->block_size :
-            256;
-        if ((block % 64) || (block > 256))
+            512;
+        if ((block % 64) || (block > 512))
                 return pg_fail(PUMAS_RETURN_VALUE_ERROR,
-                    "block size must be a multiple of 64, at most 256");
+                    "block size must be a multiple of 64, at most 512");
         const int per_cu = (options && options->blocks_per_cu > 0) ?
             options->blocks_per_cu :
-            2;
+            (512 / block > 0 ? 512 / block : 1);
         int rounds_per_check = (options && options->rounds_per_check > 0) ?
             options->rounds_per_check :
             ROUNDS_PER_CHECK;
@@ -1233,8 +1329,8 @@ extern "C" enum pumas_return pumas_gpu_transport(
         }
         const bool f32 = (precision == PUMAS_GPU_PRECISION_FP32_BULK);
         unsigned lds_bytes = 0;
-        if (f32 && !(options && options->lds_tables < 0)) {
-                rc = lds_tables_size(physics, device, per_cu, &lds_bytes);
+        if (!(options && options->lds_tables < 0)) {
+                rc = lds_tables_size(physics, device, per_cu, f32, &lds_bytes);
                 if (rc != PUMAS_RETURN_SUCCESS) return rc;
         }
         unsigned long long total_steps = 0, error = 0;
@@ -1271,6 +1367,7 @@ extern "C" enum pumas_return pumas_gpu_transport(
                 stats->steps = total_steps;
                 stats->histories = n;
                 stats->kernel_ms = ms;
+                stats->lds_bytes = lds_bytes;
         }
         if (rc != PUMAS_RETURN_SUCCESS) return rc;
         if (error != 0)

@@ -87,7 +87,7 @@ extern "C" int host_transport(const pumas_gpu_context *ctx, const pumas_gpu_geom
     c.scattering = ctx->mode.scattering; c.event = ctx->event; c.limit_energy = ctx->limit.energy;
     c.limit_distance = ctx->limit.distance; c.limit_grammage = ctx->limit.grammage; c.limit_time = ctx->limit.time;
     c.accuracy = ctx->accuracy;
-    static FloatTables g_ft; Engine eng = {g_view, g_mv, c, g, g_ft, g_rec};
+    static FloatTables g_ft; Engine eng = {g_view, g_mv, c, g, g_ft, g_view, g_rec};
     for (int64_t i = 0; i < n; i++) {
         pumas_state *s = states + i; Walker w; Track &t = w.t;
         t.charge = s->charge; t.energy = s->energy; t.distance = s->distance; t.grammage = s->grammage;

@@ -85,20 +85,32 @@ def test_fp32_forward_slabs(gpu_standard):
     assert abs(steps) < 1e-2
 
 
-@pytest.mark.parametrize("physics,tail_del", [("gpu_standard", 1),
-                                              ("gpu_standard", -1),
-                                              ("gpu_iron", 1),
-                                              ("gpu_iron", -1)])
-def test_fp32_lds_tables_invariance(request, physics, tail_del):
-    """The FP32 stepping kernel reads its tables from a copy in LDS or from
-    HBM: the same values, so the same results, bit for bit."""
+@pytest.mark.parametrize("physics", ["gpu_standard", "gpu_iron"])
+@pytest.mark.parametrize("precision", [pa.Precision.FP32_BULK,
+                                       pa.Precision.FP64])
+@pytest.mark.parametrize("tail_del", [1, -1])
+@pytest.mark.parametrize("shape", [(256, 2), (512, 1)])
+def test_lds_tables_invariance(request, physics, precision, tail_del, shape):
+    """The stepping kernel reads the tables of its per-step lookups from a
+    copy in LDS or from HBM: the same values, so the same results, bit for
+    bit, in both precisions. The FP32 mirrors fit twice in a CU's LDS (two
+    256-lane blocks); the FP64 step tables (123 KB standard rock, 163 KB
+    with iron) only once (one 512-lane block)."""
     phys = request.getfixturevalue(physics)
     s = W.slabs_forward_states(4000, seed=23)
     geom = pa.Geometry(phys, W.slabs_geometry())
     ctx = W.slabs_forward_context()
-    kw = dict(seed=23, precision=pa.Precision.FP32_BULK, tail_del=tail_del)
+    kw = dict(seed=23, precision=precision, tail_del=tail_del,
+              block_size=shape[0], blocks_per_cu=shape[1])
     hbm = pa.transport(phys, ctx, geom, s.copy(), lds_tables=-1, **kw)
     lds = pa.transport(phys, ctx, geom, s.copy(), lds_tables=1, **kw)
+    print(f"LDS copy {lds.lds_bytes} B per block")
+    assert hbm.lds_bytes == 0
+    if precision == pa.Precision.FP32_BULK or shape[1] == 1:
+        if physics == "gpu_standard":
+            assert lds.lds_bytes > 0
+    else:
+        assert lds.lds_bytes == 0
     assert (lds.states == hbm.states).all()
     for k in ("events", "media", "steps", "draws", "rc"):
         np.testing.assert_array_equal(getattr(lds, k), getattr(hbm, k))
