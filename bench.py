@@ -70,6 +70,14 @@ def parse():
                     help="skip the autotuner (default launch settings)")
     ap.add_argument("--tune-n", type=int, default=2_000_000,
                     help="histories of the autotuner's sample")
+    ap.add_argument("--max-steps", type=int, default=1_000_000,
+                    help="Monte Carlo steps after which a history ends as "
+                         "a limit (the longest histories of these "
+                         "workloads take a few thousand)")
+    ap.add_argument("--tail-del", type=int, default=0,
+                    help="DELs inside the stepping wavefronts once the step "
+                         "queue is empty: 1 always, -1 never, 0 below 2^22 "
+                         "histories")
     ap.add_argument("--lds-tables", type=int, default=0,
                     help="FP32 tables of the stepping kernel in LDS: 0 when "
                          "they fit, -1 never")
@@ -276,7 +284,8 @@ def main():
             block_size=tuned["block_size"],
             blocks_per_cu=tuned["blocks_per_cu"],
             ehs_batch=tuned["ehs_batch"], eager=not args.graph,
-            lds_tables=args.lds_tables)
+            lds_tables=args.lds_tables, max_steps=args.max_steps,
+            tail_del=args.tail_del)
 
     for k in range(args.warmup):
         st = one_pass(1000 + k)

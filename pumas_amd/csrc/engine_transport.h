@@ -85,6 +85,26 @@ struct Recorder {
 };
 
 /*
+ * The rejection and bisection loops of the reference have no iteration
+ * limit. For finite inputs each of them accepts with a probability bounded
+ * away from zero (at most 120 iterations over 10^7 voxel histories), but a
+ * NaN or infinite input can make one never accept, and a single such lane
+ * would keep its wavefront, and so the launch, from ever ending. After
+ * LOOP_MAX iterations a loop gives up: the history is marked as having used
+ * all of its steps, so that it ends at its next step boundary as the step
+ * cap ends it (EVENT_LIMIT, steps = max_steps), and the loop's last values
+ * are used once.
+ */
+constexpr int LOOP_MAX = 1 << 16;
+
+PG_FN bool loop_stalled(Track & t, int & iterations)
+{
+        if (++iterations < LOOP_MAX) return false;
+        t.steps = (uint32_t)t.max_steps;
+        return true;
+}
+
+/*
  * The transport engine. With F32 false every operation is the reference's
  * FP64 arithmetic. With F32 true (FP32-bulk) the per-step table lookups,
  * the straggling, scattering and random number sampling run in single
@@ -595,9 +615,11 @@ struct EngineT {
                         const double dk1 = sqrt(dk12);
                         if (dk0 >= 3. * dk1) {
                                 double u;
+                                int tries = 0;
                                 do
                                         u = randn(t);
-                                while (fabs(u) > 3.);
+                                while ((fabs(u) > 3.) &&
+                                    !loop_stalled(t, tries));
                                 u /= 1.015387;
                                 k1 += u * dk1;
                         } else if (dk0 >= 1.7320508 * dk1) {
@@ -717,7 +739,9 @@ struct EngineT {
                                                 end_medium = tmp_medium;
                                         s1 = s2;
                                         s2 = -step;
-                                        while (fabs(s1 - s2) > STEP_MIN) {
+                                        int halvings = 0;
+                                        while ((fabs(s1 - s2) > STEP_MIN) &&
+                                            !loop_stalled(t, halvings)) {
                                                 const double s3 =
                                                     0.5 * (s1 + s2);
                                                 position[0] = pi[0] +
@@ -1055,9 +1079,10 @@ struct EngineT {
                         double ilb1 = 0.25 * step * (invlb1 + t.step_invlb1);
                         if (ilb1 > 1.) ilb1 = 1.;
                         double mu;
+                        int tries = 0;
                         do {
                                 mu = ilb1 * bulk_neglog_uniform(t);
-                        } while (mu > 1.);
+                        } while ((mu > 1.) && !loop_stalled(t, tries));
                         rotate(t, mu);
                 }
         }
@@ -1153,9 +1178,11 @@ struct EngineT {
                         }
 
                         double zeta;
+                        int tries = 0;
                         for (;;) {
                                 zeta = t.rng.uniform() * stot;
                                 if ((zeta > 0.) && (zeta < stot)) break;
+                                if (loop_stalled(t, tries)) break;
                         }
                         /* The reference keeps the energy transfer and the
                          * weight in one union (struct del_info,
@@ -1205,6 +1232,7 @@ struct EngineT {
                 const double p0 = a0 * (Wmax - Wmin);
                 const double p2 = 1. / Wmin - 1. / Wmax;
                 double q;
+                int tries = 0;
                 for (;;) {
                         double z = t.rng.uniform() * (p0 + p2);
                         if (z <= p0) {
@@ -1217,6 +1245,7 @@ struct EngineT {
                         const double r0 = a0 + 1 / (q * q);
                         const double r1 = r0 - a1 / q;
                         if (t.rng.uniform() * r0 <= r1) break;
+                        if (loop_stalled(t, tries)) break;
                 }
                 return K - q;
         }
@@ -1327,11 +1356,16 @@ struct EngineT {
                 const double a1 = 1. - alpha;
                 const double x0 = pow(xmin, a1);
                 const double x1 = pow(xmax, a1);
+                int tries = 0;
                 for (;;) {
                         const double z = t.rng.uniform();
                         const double tmp = x0 + z * (x1 - x0);
                         r = pow(tmp, 1. / a1);
-                        if ((r < xmin) || (r >= xmax)) continue;
+                        if ((r < xmin) || (r >= xmax)) {
+                                if (!loop_stalled(t, tries)) continue;
+                                w_bias = 0.;
+                                break;
+                        }
                         w_bias = (x1 - x0) * r / (a1 * tmp);
                         break;
                 }
@@ -1520,10 +1554,15 @@ struct EngineT {
                         if (Ai < A) A = Ai;
                 }
                 double mu1;
+                int tries = 0;
                 for (;;) {
                         const double zeta = t.rng.uniform();
                         const double tmp = 1. + A - zeta * (1 - mu0);
-                        if (tmp <= 0.) continue;
+                        if (tmp <= 0.) {
+                                if (!loop_stalled(t, tries)) continue;
+                                mu1 = mu0;
+                                break;
+                        }
                         mu1 = (A + mu0) * (A + 1.) / tmp - A;
                         if (mu1 < mu0)
                                 mu1 = mu0;
@@ -1539,6 +1578,7 @@ struct EngineT {
                             mu1, d.screening[n]);
                         ratio *= ratio * (1. - d.fspin * mu1);
                         if (t.rng.uniform() <= ratio) break;
+                        if (loop_stalled(t, tries)) break;
                 }
 
                 const double gamma = d.fCM[0];
@@ -1774,6 +1814,20 @@ struct EngineT {
                         if ((w.step_max_medium > 0.) &&
                             (w.step_max_type == STEP_CHECK))
                                 w.step_max_medium += 0.5 * STEP_MIN;
+                }
+                if (!(t.energy <= DBL_MAXV) ||
+                    !(fabs(t.pos[0]) + fabs(t.pos[1]) + fabs(t.pos[2]) <=
+                        DBL_MAXV)) {
+                        /* A kinetic energy or a position that is no longer
+                         * a finite number (from a non finite or overflowing
+                         * input state) would make the history step until
+                         * the cap below, as in the reference, which has no
+                         * end for it: the history ends at once with a value
+                         * error. */
+                        w.rc = RETURN_VALUE_ERROR;
+                        t.step_event = EVENT_LIMIT;
+                        walker_finish(w);
+                        return;
                 }
                 if ((int)t.steps >= t.max_steps) {
                         /* Safety cap: end the history as a limit. */
