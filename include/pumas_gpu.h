@@ -266,8 +266,8 @@ struct pumas_gpu_options {
         const struct pumas_gpu_recorder * recorder;
         /* DELs of the last histories of a batch applied inside the
          * stepping wavefronts instead of in queued rounds: > 0 always,
-         * < 0 never, 0 for batches below 2^22 histories (latency of small
-         * requests). */
+         * < 0 never, 0 in the rounds whose histories fit in the resident
+         * lanes at once (small requests, and the end of large batches). */
         int tail_del;
         /* Tables of the per-step lookups (the FP32 mirrors in FP32-bulk,
          * the FP64 step tables otherwise) read by the stepping kernel from

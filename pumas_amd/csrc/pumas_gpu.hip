@@ -672,6 +672,8 @@ struct KernelArgs {
         int32_t * rc;
         int max_steps;
         int ehs_batch;
+        uint32_t tail_del_lanes; /* resident lanes of the stepping launch */
+        int tail_adaptive;       /* in-wavefront DELs chosen per round */
         Recorder rec;
         unsigned long long * counters;
         Walker * records;     /* [n] histories in flight */
@@ -819,8 +821,8 @@ __global__ void k_round(unsigned long long * c, int parity)
  * exhausted) applies the DELs of its histories itself instead of queueing
  * them for the next round: the last histories of a batch then run to their
  * end without a round barrier per DEL, which sets the latency of small
- * batches. The out-of-line DEL call costs registers on every step, so large
- * batches use the variant without it.
+ * batches. The out-of-line DEL call costs registers on every step, so
+ * rounds that fill the device use the variant without it.
  */
 template <bool F32, bool TAIL_DEL, bool LDS>
 __global__ void __launch_bounds__(512, 2) k_step(KernelArgs args, int parity)
@@ -849,6 +851,13 @@ __global__ void __launch_bounds__(512, 2) k_step(KernelArgs args, int parity)
         const uint32_t * in = a.q_step[parity];
         const unsigned long long n_in = a.counters[C_STEP_N0 + parity];
         if (n_in == 0) return;
+        /* Adaptive DEL mode: both variants are launched every round and
+         * the one that does not fit the round's queue returns at once. The
+         * in-wavefront DEL call costs registers on every step, so rounds
+         * that fill the resident lanes run without it; the last rounds of
+         * a batch, and small batches, run with it. */
+        if (a.tail_adaptive && (TAIL_DEL != (n_in <= a.tail_del_lanes)))
+                return;
         if constexpr (LDS) {
                 /* The copy (tens of KB to 160 KB, read from L2 by every
                  * block of the launch), in 16 B pieces. */
@@ -864,12 +873,19 @@ __global__ void __launch_bounds__(512, 2) k_step(KernelArgs args, int parity)
         w.phase = PHASE_EMPTY;
         uint32_t slot = 0;
         bool exhausted = false;
+        /* TAIL_DEL: DELs applied in the wavefront, decided when the step
+         * queue is exhausted: only if the DELs queued so far in this round
+         * would not fill the resident lanes in the next one. A full next
+         * round costs no barrier worth saving, while finishing histories in
+         * the wavefront makes this round last as long as whole histories
+         * (many DELs each at low cutoffs). */
+        bool inline_del = false;
         unsigned long long steps_acc = 0;
         for (;;) {
                 /* Hand over the histories that left the stepping phase. */
                 const bool done = w.phase == PHASE_DONE;
                 const bool to_del =
-                    (w.phase == PHASE_DEL) && !(TAIL_DEL && exhausted);
+                    (w.phase == PHASE_DEL) && !(TAIL_DEL && inline_del);
                 if (done) {
                         walker_store(a, slot, w);
                         steps_acc += w.t.steps;
@@ -891,7 +907,14 @@ __global__ void __launch_bounds__(512, 2) k_step(KernelArgs args, int parity)
                                         base = atomicAdd(
                                             a.counters + C_TAKEN, count);
                                 base = __shfl(base, leader, 64);
-                                if (base + count >= n_in) exhausted = true;
+                                if (base + count >= n_in) {
+                                        exhausted = true;
+                                        if (TAIL_DEL)
+                                                inline_del = __atomic_load_n(
+                                                    a.counters + C_DEL_N,
+                                                    __ATOMIC_RELAXED) <
+                                                    a.tail_del_lanes;
+                                }
                                 if (w.phase == PHASE_EMPTY) {
                                         const unsigned long long k = base +
                                             __popcll(free_lanes & below);
@@ -924,7 +947,7 @@ __global__ void __launch_bounds__(512, 2) k_step(KernelArgs args, int parity)
                 if (ehs && ((waiting >= batch) || !stepping)) {
                         if (w.phase == PHASE_EHS) eng.walker_ehs(w);
                 }
-                if (TAIL_DEL && exhausted && (w.phase == PHASE_DEL))
+                if (TAIL_DEL && inline_del && (w.phase == PHASE_DEL))
                         eng.walker_del(w);
         }
         steps_flush(a, steps_acc);
@@ -986,10 +1009,6 @@ namespace {
  * bytes of HBM (about 11 GB). */
 constexpr size_t MAX_CHUNK = size_t(1) << 25;
 
-/* Chunks below this many histories step with in-wavefront DELs once their
- * step queue is exhausted (k_step<F32, true>). */
-constexpr size_t TAIL_DEL_MAX = size_t(1) << 22;
-
 /* Default number of rounds launched between two reads of the queue
  * lengths, and its maximum. */
 constexpr int ROUNDS_PER_CHECK = 16;
@@ -999,7 +1018,7 @@ constexpr int MAX_ROUNDS_PER_CHECK = 256;
 struct Launch {
         int n_cu, block, per_cu, rounds_per_check;
         bool graph;
-        int tail_del; /* <0 never, >0 always, 0: below TAIL_DEL_MAX */
+        int tail_del; /* <0 never, >0 always, 0: rounds below the lanes */
         unsigned lds_bytes; /* LDS copy of the FP32 tables, or 0 */
         unsigned step_grid, del_grid;
         hipStream_t stream;
@@ -1026,10 +1045,8 @@ enum pumas_return launch_round(
         hipLaunchKernelGGL(k_round, dim3(1), dim3(64), 0, stream, a.counters,
             parity);
         if (L.marks) HIP_TRY(hipEventRecord(L.marks[2 * r], stream));
-        if (L.tail_del)
-                launch_step<F32, true>(a, parity, L);
-        else
-                launch_step<F32, false>(a, parity, L);
+        if (L.tail_del <= 0) launch_step<F32, false>(a, parity, L);
+        if (L.tail_del >= 0) launch_step<F32, true>(a, parity, L);
         if (L.marks) HIP_TRY(hipEventRecord(L.marks[2 * r + 1], stream));
         hipLaunchKernelGGL(k_del<F32>, dim3(L.del_grid), dim3(256), 0,
             stream, a, parity);
@@ -1090,10 +1107,10 @@ enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
         L.del_grid = (unsigned)((del_fill < del_max) ?
                 (del_fill > 0 ? del_fill : 1) :
                 del_max);
-        L.tail_del = (L.tail_del > 0) ||
-            ((L.tail_del == 0) && (m < TAIL_DEL_MAX));
         hipStream_t stream = L.stream;
         KernelArgs a = a0;
+        a.tail_del_lanes = L.step_grid * (uint32_t)L.block;
+        a.tail_adaptive = (L.tail_del == 0);
         a.states = a0.states + base;
         a.n = m;
         a.first = a0.first + base;
