@@ -21,7 +21,9 @@ CSDA through 1 km of standard rock, 10^8 histories (closed form, one
 muon-step per history); `terrain` — backward straggled muography along 64
 lines of sight (10^3..10^6 events each) under a 1024 x 1024 heightmap;
 `voxels` — backward straggled transport through 512^3 voxels of 2 m at a 1%
-cutoff. `--precision fp32` selects FP32-bulk.
+cutoff, 10^8 histories (10^7 generated states tiled on the device, each
+history with its own id and random stream). `--precision fp32` selects
+FP32-bulk.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N]
                        [--workload slabs|csda|terrain|voxels]
@@ -57,7 +59,7 @@ def parse():
                     choices=("slabs", "csda", "terrain", "voxels"))
     ap.add_argument("--n", type=int, default=0,
                     help="histories per pass (0: the workload's own, 10^7 "
-                         "for slabs, 2 10^6 for voxels, 10^8 for csda)")
+                         "for slabs, 10^8 for voxels and csda)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target duration of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -222,10 +224,11 @@ def setup(args, pa, W):
         states, _ = W.terrain_sightlines(64, 1023 * 5.0)
         return ("standard_c005", W.terrain_geometry(), W.terrain_context(),
                 states, len(states))
-    n = args.n or 2_000_000
-    log(f"bench: generating the 512^3 voxel grid and {n} states")
+    n = args.n or 100_000_000
+    m = min(n, 10_000_000)
+    log(f"bench: generating the 512^3 voxel grid and {m} states")
     return ("iron_c001", W.voxel_geometry_large(), W.voxel_context(),
-            W.voxel_states(n, 512 * 2.0), n)
+            W.voxel_states(m, 512 * 2.0), n)
 
 
 def main():
@@ -256,6 +259,14 @@ def main():
         reps = (n + len(base) - 1) // len(base)
         pristine = t.repeat(reps, 1)[:n].reshape(-1).contiguous()
         sample = base
+    elif len(states) < n:
+        # More histories than generated states: the states are tiled on the
+        # device. Every history has its own id, hence its own random stream,
+        # so the copies are independent histories of the same workload.
+        t = torch.from_numpy(states.view(np.uint8).reshape(-1, 104)).to(dev)
+        reps = (n + len(states) - 1) // len(states)
+        pristine = t.repeat(reps, 1)[:n].reshape(-1).contiguous()
+        sample = states
     else:
         pristine = torch.from_numpy(states.view(np.uint8).reshape(-1)).to(dev)
         sample = states
