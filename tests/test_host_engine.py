@@ -153,3 +153,28 @@ def test_recorder_frames_bit_exact(host, ref_standard_aligned, case, period):
     np.testing.assert_array_equal(f["medium"], ref["media"])
     np.testing.assert_array_equal(f["event"], ref["events"])
     np.testing.assert_array_equal(f["history"], ref["history"])
+
+
+@pytest.mark.parametrize("direction", ["forward", "backward"])
+@pytest.mark.parametrize("energy", [np.inf, np.nan])
+def test_non_finite_energy_ends_with_value_error(host, direction, energy):
+    """A history whose kinetic energy is not a finite number would step
+    until the step cap (the reference has no end for it): the engine ends it
+    at its first step with PUMAS_RETURN_VALUE_ERROR, and the other histories
+    of the batch are unaffected."""
+    if direction == "forward":
+        ctx, states = W.slabs_forward_context(), W.slabs_forward_states(4, 1)
+    else:
+        ctx = W.slabs_backward_context()
+        states = W.slabs_backward_states(4, seed=1)
+    bad = states.copy()
+    bad["energy"][1] = energy
+    s, ev, md, st, dr, rc = run_host(host, "standard_c005", ctx,
+                                     W.slabs_geometry(), bad)
+    assert rc[1] == int(pa.Return.VALUE_ERROR)
+    assert st[1] <= 1
+    ok = run_host(host, "standard_c005", ctx, W.slabs_geometry(), states)
+    for i in (0, 2, 3):
+        assert rc[i] == 0
+        assert st[i] == ok[3][i]
+        assert s["energy"][i] == ok[0]["energy"][i]
