@@ -72,6 +72,9 @@ def parse():
                     help="skip the autotuner (default launch settings)")
     ap.add_argument("--tune-n", type=int, default=2_000_000,
                     help="histories of the autotuner's sample")
+    ap.add_argument("--lines", type=int, default=4096,
+                    help="lines of sight of the terrain workload (10^3 to "
+                         "10^6 events each)")
     ap.add_argument("--max-steps", type=int, default=1_000_000,
                     help="Monte Carlo steps after which a history ends as "
                          "a limit (the longest histories of these "
@@ -198,13 +201,63 @@ WORKLOADS = {
     "csda": "backward CSDA, no scattering, 1 km of standard rock, K_f "
             "log-uniform [1e-2, 1e2] GeV",
     "terrain": "backward straggled + mixed scattering under a 1024 x 1024 "
-               "heightmap (5 m, relief 0-800 m, column density 2.0-2.9), 64 "
-               "lines of sight x 10^U[3,6] events, K_f log-uniform "
+               "heightmap (5 m, relief 0-800 m, column density 2.0-2.9), "
+               "{lines} lines of sight x 10^U[3,6] events, K_f log-uniform "
                "[1e-2, 1e2] GeV",
     "voxels": "backward straggled + mixed scattering through 512^3 voxels "
               "of 2 m (rock / water / air / iron, lognormal density), "
               "cutoff 0.01, K_f log-uniform [0.1, 1e3] GeV",
 }
+
+
+TERRAIN_EXTENT = 1023 * 5.0
+
+
+def terrain_on_device(W, torch, np, lines, n, dev, pool=10_000_000):
+    """States of the terrain workload built on the device, line by line as
+    terrain_sightlines: the detector position and direction of each event's
+    line, and a final kinetic energy log-uniform in [1e-2, 1e2] GeV from a
+    pool of `pool` Philox draws (event j takes draw j mod pool; the first
+    `pool` events are those of terrain_sightlines). Returns the states as
+    bytes on `dev` and a host sample of 2 10^5 events spread over the
+    batch."""
+    import math
+    from pumas_amd import STATE_DTYPE
+    rk = math.log(1e2 / 1e-2)
+    m = min(n, pool)
+    energy = 1e-2 * np.exp(rk * W.uniform(0, 0, np.arange(m, dtype=np.uint64),
+                                          0))
+    counts = lines["count"]
+    start = np.concatenate([[0], np.cumsum(counts)])
+
+    def fill(out, e, pos, dirs):
+        out[:, 0] = -1.0
+        out[:, 1] = e
+        out[:, 5] = e * rk
+        out[:, 6:9] = pos
+        out[:, 9:12] = dirs
+
+    e_dev = torch.from_numpy(energy).to(dev)
+    pos = torch.from_numpy(lines["position"]).to(dev)
+    dirs = torch.from_numpy(lines["direction"]).to(dev)
+    line_of = torch.repeat_interleave(
+        torch.arange(len(counts), device=dev),
+        torch.from_numpy(counts).to(dev))
+    states = torch.zeros((n, 13), dtype=torch.float64, device=dev)
+    step = 1 << 25
+    for a in range(0, n, step):
+        b = min(n, a + step)
+        j = torch.arange(a, b, device=dev)
+        li = line_of[a:b]
+        fill(states[a:b], e_dev[j % m], pos[li], dirs[li])
+    del line_of
+    idx = np.linspace(0, n - 1, 200_000).astype(np.int64)
+    li = np.searchsorted(start, idx, side="right") - 1
+    sample = np.zeros((len(idx), 13))
+    fill(sample, energy[idx % m], lines["position"][li],
+         lines["direction"][li])
+    return states.view(torch.uint8).reshape(-1), \
+        sample.view(STATE_DTYPE).reshape(-1)
 
 
 def setup(args, pa, W):
@@ -220,10 +273,15 @@ def setup(args, pa, W):
         return ("standard_c005", W.csda_rock_geometry(),
                 W.csda_rock_context(), None, n)
     if wl == "terrain":
-        log("bench: generating the 64 lines of sight")
-        states, _ = W.terrain_sightlines(64, 1023 * 5.0)
+        lines = W.terrain_lines(args.lines, TERRAIN_EXTENT)
+        n = int(lines["count"].sum())
+        log(f"bench: {args.lines} lines of sight, {n} events")
+        if n <= 10_000_000:
+            states, _ = W.terrain_sightlines(args.lines, TERRAIN_EXTENT)
+            return ("standard_c005", W.terrain_geometry(),
+                    W.terrain_context(), states, n)
         return ("standard_c005", W.terrain_geometry(), W.terrain_context(),
-                states, len(states))
+                lines, n)
     n = args.n or 100_000_000
     m = min(n, 10_000_000)
     log(f"bench: generating the 512^3 voxel grid and {m} states")
@@ -259,6 +317,9 @@ def main():
         reps = (n + len(base) - 1) // len(base)
         pristine = t.repeat(reps, 1)[:n].reshape(-1).contiguous()
         sample = base
+    elif isinstance(states, dict):
+        log("bench: building the terrain states on the device")
+        pristine, sample = terrain_on_device(W, torch, np, states, n, dev)
     elif len(states) < n:
         # More histories than generated states: the states are tiled on the
         # device. Every history has its own id, hence its own random stream,
@@ -383,7 +444,7 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {
-            "workload": WORKLOADS[args.workload],
+            "workload": WORKLOADS[args.workload].format(lines=args.lines),
             "histories": n,
             "precision": args.precision,
             "launch": tuned,

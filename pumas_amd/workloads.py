@@ -406,11 +406,8 @@ def terrain_sightlines(n_lines, extent, seed=0, nmin=1e3, nmax=1e6,
     round(10^U[log nmin, log nmax]) events of final kinetic energy
     log-uniform in [kmin, kmax] GeV. Returns (states, line index per state);
     history ids are consecutive."""
-    lines = np.arange(n_lines, dtype=np.uint64)
-    v = [uniform(seed, 6, lines, k) for k in range(5)]
-    counts = np.rint(10 ** (math.log10(nmin) + (math.log10(nmax) -
-                                                  math.log10(nmin)) * v[0]))
-    counts = counts.astype(np.int64)
+    lines = terrain_lines(n_lines, extent, seed, nmin, nmax, cmin, cmax)
+    counts = lines["count"]
     n = int(counts.sum())
     line = np.repeat(np.arange(n_lines), counts)
     ids = _ids(n, 0)
@@ -420,16 +417,28 @@ def terrain_sightlines(n_lines, extent, seed=0, nmin=1e3, nmax=1e6,
     s["charge"] = -1.0
     s["energy"] = kmin * np.exp(rk * u0)
     s["weight"] = s["energy"] * rk
-    ct = (cmin + (cmax - cmin) * v[1])[line]
-    st = np.sqrt(1.0 - ct * ct)
-    phi = (2.0 * math.pi * v[2])[line]
-    s["direction"][:, 0] = st * np.cos(phi)
-    s["direction"][:, 1] = st * np.sin(phi)
-    s["direction"][:, 2] = -ct
-    s["position"][:, 0] = (0.5 * extent * (v[3] - 0.5))[line]
-    s["position"][:, 1] = (0.5 * extent * (v[4] - 0.5))[line]
-    s["position"][:, 2] = TERRAIN_DETECTOR_Z
+    s["direction"] = lines["direction"][line]
+    s["position"] = lines["position"][line]
     return s, line
+
+
+def terrain_lines(n_lines, extent, seed=0, nmin=1e3, nmax=1e6, cmin=0.5,
+                  cmax=1.0):
+    """The lines of sight of terrain_sightlines: per line, its event count,
+    detector position and (momentum) direction."""
+    lines = np.arange(n_lines, dtype=np.uint64)
+    v = [uniform(seed, 6, lines, k) for k in range(5)]
+    counts = np.rint(10 ** (math.log10(nmin) + (math.log10(nmax) -
+                                                  math.log10(nmin)) * v[0]))
+    ct = cmin + (cmax - cmin) * v[1]
+    st = np.sqrt(1.0 - ct * ct)
+    phi = 2.0 * math.pi * v[2]
+    direction = np.stack([st * np.cos(phi), st * np.sin(phi), -ct], axis=1)
+    position = np.stack([0.5 * extent * (v[3] - 0.5),
+                         0.5 * extent * (v[4] - 0.5),
+                         np.full(n_lines, TERRAIN_DETECTOR_Z)], axis=1)
+    return {"count": counts.astype(np.int64), "direction": direction,
+            "position": position}
 
 
 def voxel_geometry_large(n=512, size=2.0, seed=0, sigma=0.1):
