@@ -116,6 +116,12 @@ struct pumas_gpu_physics {
         const double * step;    /* packed copy of the FP64 step tables */
         LdsStepTables lds_step; /* their columns, as offsets in that copy */
         uint32_t n_step;        /* doubles in that copy (multiple of 8) */
+        /* FP32-bulk LDS copy: the mirrors, then the FP64 tables of the range
+         * to energy inversion (K, X, X_dK, K_dX, a_max, b_max; the other
+         * columns of lds_range are not in it and not read). */
+        LdsStepTables lds_range;
+        uint32_t range_head;    /* doubles of K..K_dX at the head of step */
+        uint32_t range_tail_at; /* a_max..b_max: from here to n_step */
         int particle;
 };
 
@@ -393,6 +399,26 @@ extern "C" enum pumas_return pumas_gpu_physics_upload_memory(
                 ph->step = (const double *)((const uint8_t *)device +
                     step_offset);
                 ph->n_step = (uint32_t)step.size();
+
+                /* In the FP32-bulk copy, K..K_dX (the first four stepped
+                 * columns) and a_max, b_max (the last two) follow the
+                 * mirrors. */
+                static_assert(n_stepped == 17, "stepped columns");
+                ph->range_head = (uint32_t)step_at[4];
+                ph->range_tail_at = (uint32_t)step_at[15];
+                const uint32_t base = ph->n_floats / 2;
+                LdsStepTables & rt = ph->lds_range;
+                memset(&rt, 0x0, sizeof(rt));
+                rt.n_energies = st.n_energies;
+                rt.n_materials = st.n_materials;
+                rt.mass = st.mass;
+                rt.K.off = base + (uint32_t)step_at[0];
+                rt.X.off = base + (uint32_t)step_at[1];
+                rt.X_dK.off = base + (uint32_t)step_at[2];
+                rt.K_dX.off = base + (uint32_t)step_at[3];
+                rt.a_max.off = base + ph->range_head;
+                rt.b_max.off = base + ph->range_head +
+                    (uint32_t)(step_at[16] - step_at[15]);
         }
 
         ModelView & mv = ph->models;
@@ -655,10 +681,14 @@ struct KernelArgs {
         ModelView mv;
         FloatTables ft;
         LdsTables ftl;          /* ft in the stepping kernel's LDS copy */
-        uint32_t n_floats;      /* floats of that copy (from ft.K on) */
-        const double * step;    /* FP64 step tables, packed */
-        LdsStepTables stl;      /* their columns in the LDS copy */
-        uint32_t n_step;        /* doubles of that copy */
+        LdsStepTables stl;      /* FP64 step tables in the LDS copy */
+        /* The LDS copy of the stepping kernel: up to three pieces, each
+         * `n4` 16 B words from `src` to LDS word `dst`. */
+        struct {
+                const float4 * src;
+                uint32_t dst, n4;
+        } lds_piece[3];
+        int n_lds_pieces;
         Config c;
         pg_geometry g;
         struct pumas_state * states;
@@ -833,7 +863,7 @@ __global__ void __launch_bounds__(512, 2) k_step(KernelArgs args, int parity)
          * packed FP64 step tables. */
         using FT = typename std::conditional<LDS && F32, LdsTables,
             FloatTables>::type;
-        using ST = typename std::conditional<LDS && !F32, LdsStepTables,
+        using ST = typename std::conditional<LDS, LdsStepTables,
             PhysicsView>::type;
         const FT * ft;
         const ST * st;
@@ -841,7 +871,7 @@ __global__ void __launch_bounds__(512, 2) k_step(KernelArgs args, int parity)
                 ft = &a.ftl;
         else
                 ft = &a.ft;
-        if constexpr (LDS && !F32)
+        if constexpr (LDS)
                 st = &a.stl;
         else
                 st = &a.p;
@@ -860,13 +890,15 @@ __global__ void __launch_bounds__(512, 2) k_step(KernelArgs args, int parity)
                 return;
         if constexpr (LDS) {
                 /* The copy (tens of KB to 160 KB, read from L2 by every
-                 * block of the launch), in 16 B pieces. */
-                const float4 * src = F32 ? (const float4 *)a.ft.K :
-                                           (const float4 *)a.step;
-                const uint32_t n4 = F32 ? a.n_floats / 4 : a.n_step / 2;
-                float4 * dst = (float4 *)pg_lds_floats;
-                for (uint32_t i = threadIdx.x; i < n4; i += blockDim.x)
-                        dst[i] = src[i];
+                 * block of the launch), in 16 B words. */
+                float4 * lds = (float4 *)pg_lds_floats;
+                for (int k = 0; k < a.n_lds_pieces; k++) {
+                        const float4 * src = a.lds_piece[k].src;
+                        float4 * dst = lds + a.lds_piece[k].dst;
+                        for (uint32_t i = threadIdx.x; i < a.lds_piece[k].n4;
+                             i += blockDim.x)
+                                dst[i] = src[i];
+                }
                 __syncthreads();
         }
         Walker w;
@@ -1071,7 +1103,9 @@ enum pumas_return lds_tables_size(const pumas_gpu_physics * physics,
 {
         *bytes = 0;
         const size_t need = f32 ?
-            (size_t)physics->n_floats * sizeof(float) :
+            (size_t)physics->n_floats * sizeof(float) +
+                (size_t)(physics->range_head + physics->n_step -
+                    physics->range_tail_at) * sizeof(double) :
             (size_t)physics->n_step * sizeof(double);
         int per_block = 0, per_sm = 0;
         HIP_TRY(hipDeviceGetAttribute(
@@ -1270,10 +1304,23 @@ extern "C" enum pumas_return pumas_gpu_transport(
         a.mv = physics->models;
         a.ft = physics->floats;
         a.ftl = physics->lds_floats;
-        a.n_floats = physics->n_floats;
-        a.step = physics->step;
-        a.stl = physics->lds_step;
-        a.n_step = physics->n_step;
+        if (precision == PUMAS_GPU_PRECISION_FP32_BULK) {
+                a.stl = physics->lds_range;
+                const uint32_t nf4 = physics->n_floats / 4;
+                const uint32_t head4 = physics->range_head / 2;
+                a.lds_piece[0] = { (const float4 *)physics->floats.K, 0, nf4 };
+                a.lds_piece[1] = { (const float4 *)physics->step, nf4, head4 };
+                a.lds_piece[2] = { (const float4 *)(physics->step +
+                                       physics->range_tail_at),
+                        nf4 + head4,
+                        (physics->n_step - physics->range_tail_at) / 2 };
+                a.n_lds_pieces = 3;
+        } else {
+                a.stl = physics->lds_step;
+                a.lds_piece[0] = { (const float4 *)physics->step, 0,
+                        physics->n_step / 2 };
+                a.n_lds_pieces = 1;
+        }
         a.c.energy_loss = context->mode.energy_loss;
         a.c.decay = context->mode.decay;
         a.c.direction = context->mode.direction;
