@@ -905,19 +905,12 @@ __global__ void __launch_bounds__(512, 2) k_step(KernelArgs args, int parity)
         w.phase = PHASE_EMPTY;
         uint32_t slot = 0;
         bool exhausted = false;
-        /* TAIL_DEL: DELs applied in the wavefront, decided when the step
-         * queue is exhausted: only if the DELs queued so far in this round
-         * would not fill the resident lanes in the next one. A full next
-         * round costs no barrier worth saving, while finishing histories in
-         * the wavefront makes this round last as long as whole histories
-         * (many DELs each at low cutoffs). */
-        bool inline_del = false;
         unsigned long long steps_acc = 0;
         for (;;) {
                 /* Hand over the histories that left the stepping phase. */
                 const bool done = w.phase == PHASE_DONE;
                 const bool to_del =
-                    (w.phase == PHASE_DEL) && !(TAIL_DEL && inline_del);
+                    (w.phase == PHASE_DEL) && !(TAIL_DEL && exhausted);
                 if (done) {
                         walker_store(a, slot, w);
                         steps_acc += w.t.steps;
@@ -939,14 +932,7 @@ __global__ void __launch_bounds__(512, 2) k_step(KernelArgs args, int parity)
                                         base = atomicAdd(
                                             a.counters + C_TAKEN, count);
                                 base = __shfl(base, leader, 64);
-                                if (base + count >= n_in) {
-                                        exhausted = true;
-                                        if (TAIL_DEL)
-                                                inline_del = __atomic_load_n(
-                                                    a.counters + C_DEL_N,
-                                                    __ATOMIC_RELAXED) <
-                                                    a.tail_del_lanes;
-                                }
+                                if (base + count >= n_in) exhausted = true;
                                 if (w.phase == PHASE_EMPTY) {
                                         const unsigned long long k = base +
                                             __popcll(free_lanes & below);
@@ -979,7 +965,7 @@ __global__ void __launch_bounds__(512, 2) k_step(KernelArgs args, int parity)
                 if (ehs && ((waiting >= batch) || !stepping)) {
                         if (w.phase == PHASE_EHS) eng.walker_ehs(w);
                 }
-                if (TAIL_DEL && inline_del && (w.phase == PHASE_DEL))
+                if (TAIL_DEL && exhausted && (w.phase == PHASE_DEL))
                         eng.walker_del(w);
         }
         steps_flush(a, steps_acc);
