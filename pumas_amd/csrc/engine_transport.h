@@ -1518,8 +1518,15 @@ struct EngineT {
         {
                 const double kinetic = t.energy;
                 double mu0 = 0., invlb1 = 0.;
-                table_get_msc(p, t.ix, c.energy_loss, material, kinetic, &mu0,
-                    &invlb1);
+                /* The FP64 step tables (st) hold the MSC columns in FP64
+                 * mode; the FP32-bulk step view holds the range columns
+                 * only. */
+                if constexpr (F32)
+                        table_get_msc(p, t.ix, c.energy_loss, material,
+                            kinetic, &mu0, &invlb1);
+                else
+                        table_get_msc(st, t.ix, c.energy_loss, material,
+                            kinetic, &mu0, &invlb1);
 
                 const int nc = p.elements_in[material];
                 int ihard = nc - 1;
