@@ -348,9 +348,16 @@ def main():
         log(f"bench: autotuned on {m} histories: {tuned} "
             f"({rate:.3e} steps/s on the sample)")
 
+    # The plain method's tally: Gaisser-form flux at cos(theta) = 1 per cm^2,
+    # without charge fraction, summed on the device after each pass.
+    flux_spec = pa.TallySpec(flux=pa.Flux.GAISSER, charge_fraction=False,
+                             scale=1e-4, cos_from_direction=False,
+                             cos_theta=1.0) if csda else None
+    tallies = []
+
     def one_pass(k):
         work.copy_(pristine)
-        return pa.transport_device(
+        st = pa.transport_device(
             phys, ctx, geom, work.data_ptr(), n, seed=0,
             first_history=k * n, stream=stream, precision=precision,
             block_size=tuned["block_size"],
@@ -358,6 +365,10 @@ def main():
             ehs_batch=tuned["ehs_batch"], eager=not args.graph,
             lds_tables=args.lds_tables, max_steps=args.max_steps,
             tail_del=args.tail_del)
+        if flux_spec is not None:
+            tallies.append(pa.tally_device(work.data_ptr(), n, spec=flux_spec,
+                                           stream=stream))
+        return st
 
     for k in range(args.warmup):
         st = one_pass(1000 + k)
@@ -465,6 +476,12 @@ def main():
         "cpu_baseline": cpu,
         "serving": serve,
     }
+    if tallies:
+        t = tallies[-1]
+        line["tally"] = {"flux_per_cm2_s_sr": t.mean, "error": t.error,
+                         "model": "Gaisser form, cos(theta) = 1",
+                         "note": "computed on the device inside each timed "
+                                 "pass"}
     print(json.dumps(line))
 
 
