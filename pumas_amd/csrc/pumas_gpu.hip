@@ -1110,6 +1110,19 @@ enum pumas_return lds_tables_size(const pumas_gpu_physics * physics,
         return PUMAS_RETURN_SUCCESS;
 }
 
+/* Device memory and graph of a chunk, released on every return path. */
+struct ChunkResources {
+        hipStream_t stream;
+        void * mem = nullptr;
+        hipGraphExec_t exec = nullptr;
+
+        ~ChunkResources()
+        {
+                if (exec) (void)hipGraphExecDestroy(exec);
+                if (mem) (void)hipFreeAsync(mem, stream);
+        }
+};
+
 /* Transport histories [base, base + m) of the batch described by `a0`. */
 enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
     size_t m, bool f32, Launch L, unsigned long long * total_steps,
@@ -1140,13 +1153,14 @@ enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
         if (a0.draws) a.draws = a0.draws + base;
         if (a0.rc) a.rc = a0.rc + base;
 
-        void * mem = nullptr;
+        ChunkResources res;
+        res.stream = stream;
         const size_t b_counters = 256;
         const size_t b_records = ((m * sizeof(Walker)) + 255) & ~size_t(255);
         const size_t b_queue = ((m * sizeof(uint32_t)) + 255) & ~size_t(255);
         HIP_TRY(hipMallocAsync(
-            &mem, b_counters + b_records + 3 * b_queue, stream));
-        char * ptr = static_cast<char *>(mem);
+            &res.mem, b_counters + b_records + 3 * b_queue, stream));
+        char * ptr = static_cast<char *>(res.mem);
         a.counters = reinterpret_cast<unsigned long long *>(ptr);
         a.records = reinterpret_cast<Walker *>(ptr + b_counters);
         char * q = ptr + b_counters + b_records;
@@ -1165,17 +1179,14 @@ enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
         else
                 hipLaunchKernelGGL(k_init<false>, dim3(blocks), dim3(256), 0,
                     stream, a);
-        if (hipGetLastError() != hipSuccess) {
-                (void)hipFreeAsync(mem, stream);
+        if (hipGetLastError() != hipSuccess)
                 return pg_fail(PUMAS_RETURN_VALUE_ERROR,
                     "transport kernel launch failed");
-        }
         /* The sequence of rounds between two checks is captured once in a
          * HIP graph and replayed (an even number of rounds, so that every
          * replay starts on step queue 0): one launch per check instead of
          * three per round. Counters are cleared by kernels, not memset
          * nodes. */
-        hipGraphExec_t exec = nullptr;
         if (L.graph) {
                 hipGraph_t graph = nullptr;
                 HIP_TRY(hipStreamBeginCapture(
@@ -1189,12 +1200,11 @@ enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
                 const hipError_t e = hipStreamEndCapture(stream, &graph);
                 if (rc != PUMAS_RETURN_SUCCESS) {
                         if (graph) (void)hipGraphDestroy(graph);
-                        (void)hipFreeAsync(mem, stream);
                         return rc;
                 }
                 HIP_TRY(e);
                 const hipError_t ei =
-                    hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                    hipGraphInstantiate(&res.exec, graph, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(graph);
                 HIP_TRY(ei);
         }
@@ -1215,8 +1225,8 @@ enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
                 launched = 0;
                 if (host[C_STEP_N0 + parity] == 0) break;
                 if (stats) stats->rounds += 1;
-                if (exec) {
-                        HIP_TRY(hipGraphLaunch(exec, stream));
+                if (res.exec) {
+                        HIP_TRY(hipGraphLaunch(res.exec, stream));
                         launched = L.rounds_per_check;
                         continue;
                 }
@@ -1229,9 +1239,10 @@ enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
                 }
                 if (rc != PUMAS_RETURN_SUCCESS) break;
         }
-        if (exec) (void)hipGraphExecDestroy(exec);
         *total_steps += host[C_STEPS];
         if ((*error == 0) && (host[C_ERROR] != 0)) *error = host[C_ERROR];
+        void * mem = res.mem;
+        res.mem = nullptr;
         HIP_TRY(hipFreeAsync(mem, stream));
         return rc;
 }
@@ -1371,11 +1382,23 @@ extern "C" enum pumas_return pumas_gpu_transport(
                     "rounds_per_check must be at most %d",
                     MAX_ROUNDS_PER_CHECK);
 
-        hipEvent_t e0 = nullptr, e1 = nullptr;
+        /* Timing events, destroyed on every return path. */
+        struct Events {
+                hipEvent_t e0 = nullptr, e1 = nullptr;
+                std::vector<hipEvent_t> marks;
+
+                ~Events()
+                {
+                        if (e0) (void)hipEventDestroy(e0);
+                        if (e1) (void)hipEventDestroy(e1);
+                        for (auto & e : marks)
+                                if (e) (void)hipEventDestroy(e);
+                }
+        } ev;
         if (stats) {
-                HIP_TRY(hipEventCreate(&e0));
-                HIP_TRY(hipEventCreate(&e1));
-                HIP_TRY(hipEventRecord(e0, stream));
+                HIP_TRY(hipEventCreate(&ev.e0));
+                HIP_TRY(hipEventCreate(&ev.e1));
+                HIP_TRY(hipEventRecord(ev.e0, stream));
         }
         const bool f32 = (precision == PUMAS_GPU_PRECISION_FP32_BULK);
         unsigned lds_bytes = 0;
@@ -1386,12 +1409,12 @@ extern "C" enum pumas_return pumas_gpu_transport(
         unsigned long long total_steps = 0, error = 0;
         /* The legacy NULL stream cannot be captured. */
         const bool graph = !(options && options->eager) && (stream != nullptr);
-        std::vector<hipEvent_t> marks;
         if (stats) {
                 memset(stats, 0x0, sizeof(*stats));
                 if (!graph) {
-                        marks.resize(2 * rounds_per_check, nullptr);
-                        for (auto & e : marks) HIP_TRY(hipEventCreate(&e));
+                        ev.marks.resize(2 * rounds_per_check, nullptr);
+                        for (auto & e : ev.marks)
+                                HIP_TRY(hipEventCreate(&e));
                 }
         }
         /* Per-launch timing of the stepping kernel needs eager launches:
@@ -1399,21 +1422,18 @@ extern "C" enum pumas_return pumas_gpu_transport(
         const int tail_del = options ? options->tail_del : 0;
         const Launch L = { n_cu, block, per_cu, rounds_per_check, graph,
                 tail_del, lds_bytes, 0, 0, stream,
-                (stats && !graph) ? marks.data() : nullptr };
+                (stats && !graph) ? ev.marks.data() : nullptr };
         for (size_t base = 0; base < n; base += MAX_CHUNK) {
                 const size_t m = (n - base < MAX_CHUNK) ? n - base : MAX_CHUNK;
                 rc = transport_chunk(
                     a, base, m, f32, L, &total_steps, &error, stats);
                 if (rc != PUMAS_RETURN_SUCCESS) break;
         }
-        if (stats) HIP_TRY(hipEventRecord(e1, stream));
+        if (stats) HIP_TRY(hipEventRecord(ev.e1, stream));
         HIP_TRY(hipStreamSynchronize(stream));
-        for (auto & e : marks) (void)hipEventDestroy(e);
         if (stats) {
                 float ms = 0.f;
-                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-                (void)hipEventDestroy(e0);
-                (void)hipEventDestroy(e1);
+                HIP_TRY(hipEventElapsedTime(&ms, ev.e0, ev.e1));
                 stats->steps = total_steps;
                 stats->histories = n;
                 stats->kernel_ms = ms;

@@ -82,6 +82,17 @@ enum pumas_return run_host_request(const struct pumas_gpu_request & r,
         const size_t b_out = ns * (sizeof(int32_t) * 4 + sizeof(uint32_t) * 2);
         void * mem = nullptr;
         HIP_TRY(hipMallocAsync(&mem, b_states + b_out + 256, stream));
+        /* From here on, a failed copy frees the buffer before returning. */
+#define HIP_TRY_FREE(expr)                                                     \
+        do {                                                                   \
+                const hipError_t e_ = (expr);                                  \
+                if (e_ != hipSuccess) {                                        \
+                        (void)hipFreeAsync(mem, stream);                       \
+                        return pg_fail(PUMAS_RETURN_INTERNAL_ERROR,            \
+                            "%s: %s (%s:%d)", #expr, hipGetErrorString(e_),    \
+                            __FILE__, __LINE__);                               \
+                }                                                              \
+        } while (0)
         char * p = static_cast<char *>(mem);
         struct pumas_state * d_states = reinterpret_cast<struct pumas_state *>(p);
         char * q = p + ((b_states + 255) & ~size_t(255));
@@ -98,36 +109,37 @@ enum pumas_return run_host_request(const struct pumas_gpu_request & r,
         q += ns * sizeof(uint32_t);
         if (o.rc) d_out.rc = reinterpret_cast<int32_t *>(q);
 
-        HIP_TRY(hipMemcpyAsync(d_states, r.states, n * sizeof(*r.states),
+        HIP_TRY_FREE(hipMemcpyAsync(d_states, r.states, n * sizeof(*r.states),
             hipMemcpyHostToDevice, stream));
         const enum pumas_return rc = pumas_gpu_transport(r.physics,
             &r.context, r.geometry, d_states, n, r.seed, r.first_history,
             &d_out, opts, stats, stream);
         {
                 /* Results are copied back also when a history failed. */
-                HIP_TRY(hipMemcpyAsync(r.states, d_states,
+                HIP_TRY_FREE(hipMemcpyAsync(r.states, d_states,
                     n * sizeof(*r.states), hipMemcpyDeviceToHost, stream));
                 if (o.events)
-                        HIP_TRY(hipMemcpyAsync(o.events, d_out.events,
+                        HIP_TRY_FREE(hipMemcpyAsync(o.events, d_out.events,
                             n * sizeof(int32_t), hipMemcpyDeviceToHost,
                             stream));
                 if (o.media)
-                        HIP_TRY(hipMemcpyAsync(o.media, d_out.media,
+                        HIP_TRY_FREE(hipMemcpyAsync(o.media, d_out.media,
                             2 * n * sizeof(int32_t), hipMemcpyDeviceToHost,
                             stream));
                 if (o.steps)
-                        HIP_TRY(hipMemcpyAsync(o.steps, d_out.steps,
+                        HIP_TRY_FREE(hipMemcpyAsync(o.steps, d_out.steps,
                             n * sizeof(uint32_t), hipMemcpyDeviceToHost,
                             stream));
                 if (o.draws)
-                        HIP_TRY(hipMemcpyAsync(o.draws, d_out.draws,
+                        HIP_TRY_FREE(hipMemcpyAsync(o.draws, d_out.draws,
                             n * sizeof(uint32_t), hipMemcpyDeviceToHost,
                             stream));
                 if (o.rc)
-                        HIP_TRY(hipMemcpyAsync(o.rc, d_out.rc,
+                        HIP_TRY_FREE(hipMemcpyAsync(o.rc, d_out.rc,
                             n * sizeof(int32_t), hipMemcpyDeviceToHost,
                             stream));
         }
+#undef HIP_TRY_FREE
         HIP_TRY(hipFreeAsync(mem, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         return rc;
@@ -206,10 +218,12 @@ extern "C" enum pumas_return pumas_gpu_server_create(
         if (server == NULL)
                 return pg_fail(PUMAS_RETURN_VALUE_ERROR, "NULL argument");
         *server = nullptr;
-        auto * s = new pumas_gpu_server();
-        HIP_TRY(hipGetDevice(&s->device));
+        int device = 0;
+        HIP_TRY(hipGetDevice(&device));
         size_t free_b = 0, total_b = 0;
         HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+        auto * s = new pumas_gpu_server();
+        s->device = device;
         s->budget = (options && options->hbm_budget > 0) ?
             options->hbm_budget :
             size_t(200) << 30;
@@ -307,7 +321,9 @@ extern "C" enum pumas_return pumas_gpu_server_wait(pumas_gpu_server * s,
                             (unsigned long long)ticket);
                 job = it->second;
                 s->cv_done.wait(lock, [job] { return job->done; });
-                s->jobs.erase(it);
+                /* By key: submissions made while waiting may have rehashed
+                 * the map and invalidated `it`. */
+                s->jobs.erase(ticket);
         }
         if (status) *status = job->status;
         const enum pumas_return rc = job->status.rc;

@@ -11,11 +11,13 @@
  * (flux.c:70-89). The estimator is the one of examples/pumas/straight.c
  * (a weighted mean over histories).
  *
- * Sums are deterministic: a block reduces its histories in a fixed order
- * (each bin owned by one thread, scanning the block's values in index
- * order), blocks write partial sums, and a second kernel adds the partials
- * in block order. Results therefore do not depend on scheduling, on other
- * work running on the device, or on the run.
+ * Sums are deterministic: each thread accumulates its own histories (a
+ * fixed set for a given batch size) in index order, a block adds its
+ * threads' sums by a fixed pairwise tree, histogram bins are owned by one
+ * thread each and scan the block's values in index order, blocks write
+ * partial sums, and a second kernel adds the partials in block order.
+ * Results therefore do not depend on scheduling, on other work running on
+ * the device, or on the run.
  */
 #include <hip/hip_runtime.h>
 
@@ -135,21 +137,19 @@ __global__ void __launch_bounds__(TALLY_BLOCK) k_tally(
                                 }
                         }
                 }
+                if (sel) {
+                        t_sum += v;
+                        t_sum2 += v * v;
+                        t_count += 1.;
+                }
+                if ((a.n_energy <= 0) && (a.n_cos <= 0)) continue;
+                /* Histograms: thread j owns the j-th energy and cos bins
+                 * and scans the block's values in index order. */
                 s_v[j] = v;
                 s_sel[j] = sel;
                 s_be[j] = be;
                 s_bc[j] = bc;
                 __syncthreads();
-                /* Fixed order reductions: thread 0 the totals, thread j the
-                 * j-th energy and cos bins. */
-                if (j == 0) {
-                        for (int k = 0; k < TALLY_BLOCK; k++) {
-                                if (!s_sel[k]) continue;
-                                t_sum += s_v[k];
-                                t_sum2 += s_v[k] * s_v[k];
-                                t_count += 1.;
-                        }
-                }
                 if (j < a.n_energy) {
                         for (int k = 0; k < TALLY_BLOCK; k++) {
                                 if (s_be[k] != j) continue;
@@ -166,11 +166,25 @@ __global__ void __launch_bounds__(TALLY_BLOCK) k_tally(
                 }
                 __syncthreads();
         }
+        /* Totals: the threads' sums added by a fixed pairwise tree. */
+        __shared__ double s_t[3][TALLY_BLOCK];
+        s_t[0][j] = t_sum;
+        s_t[1][j] = t_sum2;
+        s_t[2][j] = t_count;
+        __syncthreads();
+        for (int half = TALLY_BLOCK / 2; half > 0; half >>= 1) {
+                if (j < half) {
+                        s_t[0][j] += s_t[0][j + half];
+                        s_t[1][j] += s_t[1][j + half];
+                        s_t[2][j] += s_t[2][j + half];
+                }
+                __syncthreads();
+        }
         double * out = partial + (size_t)blockIdx.x * a.n_out;
         if (j == 0) {
-                out[0] = t_sum;
-                out[1] = t_sum2;
-                out[2] = t_count;
+                out[0] = s_t[0][0];
+                out[1] = s_t[1][0];
+                out[2] = s_t[2][0];
         }
         if (j < a.n_energy) {
                 out[3 + 2 * j] = e_sum;
