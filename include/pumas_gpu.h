@@ -276,6 +276,12 @@ struct pumas_gpu_options {
          * then read from HBM through the caches). Results do not depend on
          * it. */
         int lds_tables;
+        /* Histories per wavefront of the stepping kernel (1 to 64). 0: a
+         * batch that does not fill the resident wavefronts is spread over
+         * all of them with fewer histories each (lower latency for small
+         * requests); 64: full wavefronts on as few CUs as the batch fills.
+         * Results do not depend on it. */
+        int lanes_per_wave;
 };
 
 /*

@@ -294,7 +294,7 @@ class TransportStats:
 
 def _options(precision, block_size, blocks_per_cu, max_steps,
              rounds_per_check=0, eager=False, ehs_batch=0, recorder=None,
-             tail_del=0, lds_tables=0):
+             tail_del=0, lds_tables=0, lanes_per_wave=0):
     o = _abi.GpuOptions()
     o.precision = int(precision)
     o.block_size = block_size
@@ -306,6 +306,7 @@ def _options(precision, block_size, blocks_per_cu, max_steps,
     o.recorder = recorder.handle if recorder is not None else None
     o.tail_del = int(tail_del)
     o.lds_tables = int(lds_tables)
+    o.lanes_per_wave = int(lanes_per_wave)
     return o
 
 
@@ -313,7 +314,7 @@ def transport(physics, context, geometry, states, seed=0, first_history=0,
               outputs=True, precision=Precision.FP64, block_size=0,
               blocks_per_cu=0, max_steps=0, raise_errors=True,
               rounds_per_check=0, eager=False, ehs_batch=0, recorder=None,
-              tail_del=0, lds_tables=0):
+              tail_del=0, lds_tables=0, lanes_per_wave=0):
     """Transport host `states` (a STATE_DTYPE array, updated in place).
 
     Batch counterpart of pumas_context_transport: history i draws from the
@@ -342,7 +343,7 @@ def transport(physics, context, geometry, states, seed=0, first_history=0,
     stats = _abi.GpuStats()
     opts = _options(precision, block_size, blocks_per_cu, max_steps,
                     rounds_per_check, eager, ehs_batch, recorder, tail_del,
-                    lds_tables)
+                    lds_tables, lanes_per_wave)
     ctx = context.to_c()
     rc = lib.pumas_gpu_transport_host(
         physics.handle, ctypes.byref(ctx), geometry.handle,
@@ -362,7 +363,7 @@ def transport_device(physics, context, geometry, states_ptr, n, seed=0,
                      block_size=0, blocks_per_cu=0, max_steps=0, stream=None,
                      stats=True, rounds_per_check=0, eager=False,
                      ehs_batch=0, recorder=None, tail_del=0,
-                     lds_tables=0):
+                     lds_tables=0, lanes_per_wave=0):
     """Transport `n` states already in device memory at `states_ptr`.
 
     `outputs` maps any of "events", "media", "steps", "draws", "rc" to a
@@ -376,7 +377,7 @@ def transport_device(physics, context, geometry, states_ptr, n, seed=0,
     st = _abi.GpuStats()
     opts = _options(precision, block_size, blocks_per_cu, max_steps,
                     rounds_per_check, eager, ehs_batch, recorder, tail_del,
-                    lds_tables)
+                    lds_tables, lanes_per_wave)
     ctx = context.to_c()
     _check(lib.pumas_gpu_transport(
         physics.handle, ctypes.byref(ctx), geometry.handle,

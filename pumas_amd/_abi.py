@@ -148,7 +148,8 @@ class GpuOptions(ctypes.Structure):
                 ("blocks_per_cu", ctypes.c_int), ("max_steps", ctypes.c_int),
                 ("rounds_per_check", ctypes.c_int), ("eager", ctypes.c_int),
                 ("ehs_batch", ctypes.c_int), ("recorder", ctypes.c_void_p),
-                ("tail_del", ctypes.c_int), ("lds_tables", ctypes.c_int)]
+                ("tail_del", ctypes.c_int), ("lds_tables", ctypes.c_int),
+                ("lanes_per_wave", ctypes.c_int)]
 
 
 class GpuStats(ctypes.Structure):

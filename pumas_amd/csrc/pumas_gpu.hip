@@ -703,6 +703,7 @@ struct KernelArgs {
         int max_steps;
         int ehs_batch;
         uint32_t tail_del_lanes; /* resident lanes of the stepping launch */
+        int lanes_per_wave;      /* lanes of a wavefront taking histories */
         int tail_adaptive;       /* in-wavefront DELs chosen per round */
         Recorder rec;
         unsigned long long * counters;
@@ -878,6 +879,10 @@ __global__ void __launch_bounds__(512, 2) k_step(KernelArgs args, int parity)
         const EngineT<F32, FT, ST> eng = { a.p, a.mv, a.c, a.g, *ft, *st,
                 a.rec };
         const uint64_t below = lanes_below();
+        /* Lanes of the wavefront that take histories: all of them, or the
+         * first lanes_per_wave when a small batch is spread over the
+         * device. */
+        const bool lane_on = (int)(threadIdx.x & 63) < a.lanes_per_wave;
         const uint32_t * in = a.q_step[parity];
         const unsigned long long n_in = a.counters[C_STEP_N0 + parity];
         if (n_in == 0) return;
@@ -922,7 +927,7 @@ __global__ void __launch_bounds__(512, 2) k_step(KernelArgs args, int parity)
                 /* Refill the free lanes from the step queue. */
                 if (!exhausted) {
                         const uint64_t free_lanes =
-                            __ballot(w.phase == PHASE_EMPTY);
+                            __ballot((w.phase == PHASE_EMPTY) && lane_on);
                         if (free_lanes) {
                                 const int count = __popcll(free_lanes);
                                 const int leader = __ffsll(
@@ -933,7 +938,7 @@ __global__ void __launch_bounds__(512, 2) k_step(KernelArgs args, int parity)
                                             a.counters + C_TAKEN, count);
                                 base = __shfl(base, leader, 64);
                                 if (base + count >= n_in) exhausted = true;
-                                if (w.phase == PHASE_EMPTY) {
+                                if ((w.phase == PHASE_EMPTY) && lane_on) {
                                         const unsigned long long k = base +
                                             __popcll(free_lanes & below);
                                         if (k < n_in) {
@@ -1037,7 +1042,8 @@ struct Launch {
         int n_cu, block, per_cu, rounds_per_check;
         bool graph;
         int tail_del; /* <0 never, >0 always, 0: rounds below the lanes */
-        unsigned lds_bytes; /* LDS copy of the FP32 tables, or 0 */
+        int lanes_per_wave; /* 1..64, or 0: spread small batches */
+        unsigned lds_bytes; /* LDS copy of the step tables, or 0 */
         unsigned step_grid, del_grid;
         hipStream_t stream;
         hipEvent_t * marks; /* [2 * rounds_per_check] or NULL */
@@ -1131,8 +1137,18 @@ enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
         /* No more resident blocks than the chunk fills with histories, so
          * that a small request takes full wavefronts on few CUs and leaves
          * the rest of the device to concurrent requests. */
-        const size_t fill = (m + L.block - 1) / L.block;
         const size_t resident = (size_t)L.n_cu * L.per_cu;
+        const size_t waves = (size_t)(L.block / 64);
+        int lanes = L.lanes_per_wave;
+        if (lanes <= 0) {
+                /* Spread: as many histories per wavefront as it takes for
+                 * the batch to fill every resident wavefront. */
+                const size_t per = (m + resident * waves - 1) /
+                    (resident * waves);
+                lanes = (per < 1) ? 1 : (per > 64 ? 64 : (int)per);
+        }
+        const size_t per_block = waves * (size_t)lanes;
+        const size_t fill = (m + per_block - 1) / per_block;
         L.step_grid = (unsigned)((fill < resident) ? (fill > 0 ? fill : 1) :
                                                      resident);
         const size_t del_fill = (m + 255) / 256;
@@ -1142,7 +1158,8 @@ enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
                 del_max);
         hipStream_t stream = L.stream;
         KernelArgs a = a0;
-        a.tail_del_lanes = L.step_grid * (uint32_t)L.block;
+        a.lanes_per_wave = lanes;
+        a.tail_del_lanes = L.step_grid * (uint32_t)(waves * lanes);
         a.tail_adaptive = (L.tail_del == 0);
         a.states = a0.states + base;
         a.n = m;
@@ -1420,8 +1437,13 @@ extern "C" enum pumas_return pumas_gpu_transport(
         /* Per-launch timing of the stepping kernel needs eager launches:
          * events recorded inside a graph cannot be timed. */
         const int tail_del = options ? options->tail_del : 0;
+        int lanes_per_wave = options ? options->lanes_per_wave : 0;
+        if (lanes_per_wave > 64)
+                return pg_fail(PUMAS_RETURN_VALUE_ERROR,
+                    "lanes_per_wave must be at most 64 (got %d)",
+                    lanes_per_wave);
         const Launch L = { n_cu, block, per_cu, rounds_per_check, graph,
-                tail_del, lds_bytes, 0, 0, stream,
+                tail_del, lanes_per_wave, lds_bytes, 0, 0, stream,
                 (stats && !graph) ? ev.marks.data() : nullptr };
         for (size_t base = 0; base < n; base += MAX_CHUNK) {
                 const size_t m = (n - base < MAX_CHUNK) ? n - base : MAX_CHUNK;

@@ -58,6 +58,7 @@ struct Job {
 struct pumas_gpu_server {
         int device;
         size_t budget, in_use;
+        int running; /* requests admitted and not done */
         int slots;
         std::mutex mu;
         std::condition_variable cv_work, cv_done;
@@ -152,6 +153,7 @@ void worker_loop(pumas_gpu_server * s)
         (void)hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
         for (;;) {
                 Job * job = nullptr;
+                int job_lanes = 64;
                 {
                         std::unique_lock<std::mutex> lock(s->mu);
                         s->cv_work.wait(lock, [s] {
@@ -163,6 +165,14 @@ void worker_loop(pumas_gpu_server * s)
                         job = s->pending.front();
                         s->pending.pop_front();
                         s->in_use += job->bytes;
+                        s->running += 1;
+                        /* Alone on the device, a small request is spread
+                         * over every CU (latency); with others in flight or
+                         * waiting it takes full wavefronts, which leaves
+                         * the rest of the device to them (throughput). */
+                        job_lanes = ((s->running > 1) || !s->pending.empty()) ?
+                            64 :
+                            0;
                         job->t_admit = Clock::now();
                 }
                 /* Admitting one may let the next fit as well. */
@@ -171,6 +181,7 @@ void worker_loop(pumas_gpu_server * s)
                 struct pumas_gpu_options opts;
                 memset(&opts, 0x0, sizeof(opts));
                 opts.precision = job->req.precision;
+                opts.lanes_per_wave = job_lanes;
                 struct pumas_gpu_stats stats;
                 memset(&stats, 0x0, sizeof(stats));
                 enum pumas_return rc;
@@ -201,6 +212,7 @@ void worker_loop(pumas_gpu_server * s)
                                     sizeof(job->status.message) - 1);
                         job->done = true;
                         s->in_use -= job->bytes;
+                        s->running -= 1;
                 }
                 s->cv_done.notify_all();
                 s->cv_work.notify_all();
@@ -235,6 +247,7 @@ extern "C" enum pumas_return pumas_gpu_server_create(
                     "at most %d server slots", PUMAS_GPU_SERVER_MAX_SLOTS);
         }
         s->in_use = 0;
+        s->running = 0;
         s->next_ticket = 1;
         s->stop = false;
         for (int i = 0; i < s->slots; i++)

@@ -259,3 +259,19 @@ def test_autotune(gpu_standard):
     tuned = pa.transport(gpu_standard, ctx, geom, s.copy(), **best)
     ref = pa.transport(gpu_standard, ctx, geom, s.copy())
     assert (tuned.states == ref.states).all()
+
+
+@pytest.mark.parametrize("lanes", [1, 7, 64])
+def test_lanes_per_wave_invariance(gpu_standard, lanes):
+    """Spreading a batch over more wavefronts with fewer histories each
+    changes where histories run, never their results."""
+    s = W.slabs_forward_states(3000, seed=17)
+    geom = pa.Geometry(gpu_standard, W.slabs_geometry())
+    ctx = W.slabs_forward_context()
+    ref = pa.transport(gpu_standard, ctx, geom, s.copy(), seed=17,
+                       lanes_per_wave=64)
+    alt = pa.transport(gpu_standard, ctx, geom, s.copy(), seed=17,
+                       lanes_per_wave=lanes)
+    assert (alt.states == ref.states).all()
+    for k in ("events", "media", "steps", "draws", "rc"):
+        np.testing.assert_array_equal(getattr(alt, k), getattr(ref, k))
