@@ -358,8 +358,9 @@ struct EngineT {
                     cos_theta * direction[2] + st * (cp * u0z + sp * u1z);
         }
 
-        /* transport_with_csda, src/pumas.c:4777-4949 (no magnetic field). */
-        PG_NOINLINE int transport_with_csda(
+        /* transport_with_csda, src/pumas.c:4777-4949 (no magnetic field).
+         * Inlined into the entry kernel, its only caller. */
+        PG_FN int transport_with_csda(
             Track & t, int material, double density) const
         {
                 const double ki = t.energy;
@@ -1615,10 +1616,12 @@ struct EngineT {
          * scheduler may interleave the phases of many histories freely.
          *
          * The phase functions are inlined into the scheduler kernels so
-         * that a Walker stays in registers. The rare, heavy parts (closed
-         * form CSDA and DEL sampling) are out of line and work on copies of
-         * the Track and of the engine: passing the address of the Walker or
-         * of the kernel's engine to a call would pin them to scratch memory.
+         * that a Walker stays in registers. The rare, heavy DEL sampling is
+         * out of line and works on copies of the Track and of the engine:
+         * passing the address of the Walker or of the kernel's engine to a
+         * call would pin them to scratch memory. The closed form CSDA runs
+         * only in the entry kernel (one lane per history, not persistent)
+         * and is inlined there.
          */
         PG_FN void walker_init(Walker & w) const
         {
@@ -1692,11 +1695,7 @@ struct EngineT {
                 w.end_medium = medium;
                 if (!do_stepping) {
                         record_frame(w, medium, EVENT_START);
-                        Track tmp = t;
-                        const EngineT cold = *this;
-                        w.event = cold.transport_with_csda(
-                            tmp, material, density);
-                        t = tmp;
+                        w.event = transport_with_csda(t, material, density);
                         record_frame(w, medium, w.event | EVENT_STOP);
                         return;
                 }
