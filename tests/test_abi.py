@@ -67,3 +67,54 @@ def test_transport_null_arguments():
     rc = lib.pumas_gpu_transport(None, ctypes.byref(ctx), None, None, 0, 0,
                                  0, None, None, None, None)
     assert rc == pa.Return.VALUE_ERROR
+
+
+def test_bench_refuses_several_gpus():
+    """One GPU per box here: bench.py --gpus N > 1 exits with a message and
+    a non zero status instead of launching anything."""
+    import json
+    import subprocess
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    r = subprocess.run([sys.executable, os.path.join(here, "..", "bench.py"),
+                        "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
+
+
+def test_ctypes_mirrors_match_the_header(tmp_path):
+    """The ctypes structures of the binding have the sizes and member
+    offsets that the C compiler gives the header's structures."""
+    checks = {
+        "GpuOptions": ("struct pumas_gpu_options",
+                       ["precision", "recorder", "tail_del", "lds_tables",
+                        "lanes_per_wave"]),
+        "GpuStats": ("struct pumas_gpu_stats", ["rounds", "lds_bytes"]),
+        "GpuOutputs": ("struct pumas_gpu_outputs", ["rc"]),
+        "TallySpec": ("struct pumas_gpu_tally_spec", ["cos_max"]),
+        "RequestStatus": ("struct pumas_gpu_request_status", ["message"]),
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>',
+             '#include "pumas_gpu.h"', 'int main(void) {']
+    for name, (cname, fields) in checks.items():
+        lines.append(f'printf("{name} size %zu\\n", sizeof({cname}));')
+        for f in fields:
+            lines.append(f'printf("{name} {f} %zu\\n", '
+                         f'offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.check_call(["gcc", "-std=c99", "-I",
+                           os.path.join(REPO, "include"), "-o", str(exe),
+                           str(src)])
+    out = subprocess.check_output([str(exe)], text=True).split("\n")
+    for line in filter(None, out):
+        name, what, value = line.split()
+        cls = getattr(_abi, name)
+        if what == "size":
+            assert ctypes.sizeof(cls) == int(value), line
+        else:
+            assert getattr(cls, what).offset == int(value), line
