@@ -295,9 +295,9 @@ def main():
         copy_calibration(args.copy_mib)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU "
-                          "sharding is described in DESIGN.md and not "
-                          "launched"}))
+        print(json.dumps({"error": "runs on one GPU only (--gpus 1); "
+                          "batches would shard across GPUs by history id "
+                          "range, which is not launched here"}))
         sys.exit(2)
 
     import numpy as np

@@ -61,8 +61,7 @@ def library():
         if not os.path.exists(LIBRARY_PATH):
             raise ImportError(
                 f"{LIBRARY_PATH} is missing: build it with "
-                "`python -c 'import __graft_entry__ as g; g.build()'` "
-                "(or `make -C pumas_amd/csrc`)")
+                "`make -C pumas_amd/csrc`")
         lib = ctypes.CDLL(LIBRARY_PATH)
         _abi.declare(lib)
         _lib = lib
