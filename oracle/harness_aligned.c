@@ -3,8 +3,8 @@
  * rounded double precision logarithm / exponential, the evaluation the
  * device engine uses for the FP32 DCS splines (src/pumas.c:11298-11346).
  * Everything else is the unmodified reference (src/pumas.c). Comparing the
- * device with this build isolates the C library's float rounding from the
- * transport algorithm.
+ * device with this variant of the oracle isolates the C library's float
+ * rounding from the transport algorithm.
  */
 #include <math.h>
 
