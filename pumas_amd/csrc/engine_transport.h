@@ -1515,6 +1515,78 @@ struct EngineT {
          * again instead of being kept for all elements; the recomputation
          * repeats the same operations on the same inputs.
          */
+        /* The rejection sampling of transport_do_ehs (src/pumas.c:
+         * 5834-5864) in single precision, for the FP32-bulk engine: the same
+         * draws and acceptance test, with 1 + A - zeta (1 - mu0) summed as
+         * (1 - zeta) + A + zeta mu0 (no cancellation in float) and the
+         * nuclear form factor by its series below x = 0.5. The screening
+         * set up and the CM to lab transform stay in double precision. */
+        PG_FN double ehs_sample_f32(
+            Track & t, const CoulombData & d, double A, double mu0) const
+        {
+                const int n = d.n_parameters - 1;
+                const float Af = (float)A, mu0f = (float)mu0;
+                const float fspin = (float)d.fspin;
+                const float N = (float)d.screening[n];
+                float amp[3], scr[3];
+                for (int i = 0; i < 3; i++) {
+                        amp[i] = (i < n) ? (float)d.amplitude[i] : 0.f;
+                        scr[i] = (i < n) ? (float)d.screening[i] : 1.f;
+                }
+                float mu1 = mu0f;
+                int tries = 0;
+                for (;;) {
+                        const float zeta = t.rng.uniformf();
+                        const float tmp = (1.f - zeta) + Af + zeta * mu0f;
+                        if (tmp <= 0.f) {
+                                if (!loop_stalled(t, tries)) continue;
+                                mu1 = mu0f;
+                                break;
+                        }
+                        mu1 = (Af + mu0f) * (Af + 1.f) / tmp - Af;
+                        if (mu1 < mu0f)
+                                mu1 = mu0f;
+                        else if (mu1 > 1.f)
+                                mu1 = 1.f;
+
+                        float ratio = 0.f;
+                        for (int i = 0; i < 3; i++) {
+                                if (i < n)
+                                        ratio += amp[i] * (Af + mu1) /
+                                            (scr[i] + mu1);
+                        }
+                        ratio *= ehs_form_factor_f32(mu1, N);
+                        ratio *= ratio * (1.f - fspin * mu1);
+                        if (t.rng.uniformf() <= ratio) break;
+                        if (loop_stalled(t, tries)) break;
+                }
+                return mu1;
+        }
+
+        /* coulomb_nuclear_form_factor (src/pumas.c:7396-7417) in single
+         * precision: 3 (sin x - x cos x) / x^3 squared, by its Taylor
+         * series below x = 0.5 (where the difference cancels). */
+        static PG_FN float ehs_form_factor_f32(float mu, float N)
+        {
+                if (mu <= 0.f) return 1.f;
+                if (mu > 1.f) return 0.f;
+                const float x2 = 10.f * mu / N;
+                float d;
+                if (x2 <= 1E-04f) {
+                        d = 1.f / (1.f + 0.1f * x2);
+                } else if (x2 < 0.25f) {
+                        d = 1.f + x2 * (-1.f / 10.f + x2 * (1.f / 280.f +
+                                      x2 * (-1.f / 15120.f +
+                                               x2 * (1.f / 1330560.f))));
+                } else {
+                        const float x = sqrtf(x2);
+                        float sx, cx;
+                        sincosf(x, &sx, &cx);
+                        d = 3.f * (sx - x * cx) / (x2 * x);
+                }
+                return d * d;
+        }
+
         PG_FN void transport_do_ehs(Track & t, int material) const
         {
                 const double kinetic = t.energy;
@@ -1562,6 +1634,9 @@ struct EngineT {
                         if (Ai < A) A = Ai;
                 }
                 double mu1;
+                if constexpr (F32)
+                        mu1 = ehs_sample_f32(t, d, A, mu0);
+                else {
                 int tries = 0;
                 for (;;) {
                         const double zeta = t.rng.uniform();
@@ -1587,6 +1662,7 @@ struct EngineT {
                         ratio *= ratio * (1. - d.fspin * mu1);
                         if (t.rng.uniform() <= ratio) break;
                         if (loop_stalled(t, tries)) break;
+                }
                 }
 
                 const double gamma = d.fCM[0];
