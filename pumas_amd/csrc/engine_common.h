@@ -18,10 +18,12 @@
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define PG_FN __device__ __forceinline__
+#define PG_HD __host__ __device__ __forceinline__
 #define PG_NOINLINE __device__ __noinline__
 #else
 #include <math.h>
 #define PG_FN inline
+#define PG_HD inline
 #define PG_NOINLINE __attribute__((noinline))
 #endif
 
@@ -211,6 +213,11 @@ struct PhysicsView {
         const float *DCS, *DCS_x, *DCS_envelope;
         const int *elements_in, *component_element, *element_index;
         const int *material_component0; /* first component of a material */
+        /* Coarse index of the kinetic energy grid (see k_index_fast), or
+         * NULL. Lookups through this view use it on a far miss. */
+        const int16_t *k_coarse;
+        int k_bin0, k_nbins;
+        static constexpr bool coarse_index = true;
 };
 
 /* Transport configuration (value part of struct pumas_context). */
@@ -305,13 +312,63 @@ struct Cache {
 };
 
 /*
+ * Coarse index of the kinetic energy grid: the positive doubles are binned
+ * by their exponent and three leading mantissa bits (bins of 1/8 octave,
+ * narrower than the grid's rows), and bin b holds the bracketing row of its
+ * lower edge. The bracketing row of x is then that of its bin or one of the
+ * next rows, found by a few comparisons instead of a bisection; the result
+ * is the same row, the grid being increasing.
+ */
+PG_HD int k_bin(double x)
+{
+        return (int)(__builtin_bit_cast(uint64_t, x) >> 49);
+}
+
+/* Bracketing row of K[0] <= x < K[n - 1] from the coarse index, or -1. */
+template <class V>
+PG_FN int k_index_fast(const V & p, double x)
+{
+        if (p.k_coarse == nullptr) return -1;
+        const int b = k_bin(x) - p.k_bin0;
+        if ((b < 0) || (b >= p.k_nbins)) return -1;
+        int i = p.k_coarse[b];
+        for (int k = 0; k < 4; k++, i++)
+                if (x < p.K[i + 1]) return i;
+        return -1;
+}
+
+/* Builds the coarse index of the grid K[0..n) into out[max_bins] (host
+ * side, at table upload). Returns the number of bins, 0 if they do not fit
+ * (the lookups then bisect). */
+inline int k_coarse_build(
+    const double * K, int n, int16_t * out, int max_bins, int * bin0)
+{
+        if ((n < 3) || !(K[1] > 0.)) return 0;
+        const int b0 = k_bin(K[1]);
+        const int nb = k_bin(K[n - 1]) - b0 + 1;
+        if ((nb <= 0) || (nb > max_bins)) return 0;
+        int i = 0;
+        for (int b = 0; b < nb; b++) {
+                const double edge =
+                    __builtin_bit_cast(double, (uint64_t)(b0 + b) << 49);
+                while ((i + 1 < n - 1) && (K[i + 1] <= edge)) i++;
+                out[b] = (int16_t)i;
+        }
+        *bin0 = b0;
+        return nb;
+}
+
+/*
  * Bracketing row of a value missed by the first cached row. The second
  * cached row and the two rows next to the first one are checked together
  * (one round trip to memory): between two lookups of a history the value
- * usually moves by less than a row. The bisection is the fallback.
+ * usually moves by less than a row. Then the coarse index when `kp` (the
+ * tables, for a lookup on their kinetic energy grid) has one, and the
+ * bisection last.
  */
-template <class C>
-PG_FN int cached_index_miss(C table, int n, double value, int * last)
+template <class C, class V = PhysicsView>
+PG_FN int cached_index_miss(
+    C table, int n, double value, int * last, const V * kp = nullptr)
 {
         const int i0 = last[0];
         const double below = table[(i0 > 0) ? i0 - 1 : 0];
@@ -326,8 +383,13 @@ PG_FN int cached_index_miss(C table, int n, double value, int * last)
         } else if ((i0 + 2 < n) && (value >= hi) && (value < above)) {
                 i1 = i0 + 1;
         } else {
-                i1 = table_index(table, n, value);
-                if ((i1 < 0) || (i1 >= n - 1)) return i1;
+                i1 = -1;
+                if (kp && (value >= table[0]) && (value < table[n - 1]))
+                        i1 = k_index_fast(*kp, value);
+                if (i1 < 0) {
+                        i1 = table_index(table, n, value);
+                        if ((i1 < 0) || (i1 >= n - 1)) return i1;
+                }
         }
         if (i1 != last[0])
                 last[0] = i1;
@@ -336,19 +398,33 @@ PG_FN int cached_index_miss(C table, int n, double value, int * last)
         return i1;
 }
 
-template <class C>
-PG_FN int cached_index(C table, int n, double value, int * last)
+template <class C, class V = PhysicsView>
+PG_FN int cached_index(
+    C table, int n, double value, int * last, const V * kp = nullptr)
 {
         if ((value >= table[last[0]]) && (value < table[last[0] + 1]))
                 return last[0];
-        return cached_index_miss(table, n, value, last);
+        return cached_index_miss(table, n, value, last, kp);
 }
 
 /* Bracketing row in the kinetic energy grid. */
 template <class V>
 PG_FN int index_K(const V & p, Cache & ch, double kinetic)
 {
-        return cached_index(p.K, p.n_energies, kinetic, ch.k);
+        if constexpr (V::coarse_index)
+                return cached_index(p.K, p.n_energies, kinetic, ch.k, &p);
+        else
+                return cached_index(p.K, p.n_energies, kinetic, ch.k);
+}
+
+/* Far miss on the kinetic energy grid of `p`. */
+template <class V>
+PG_FN int k_miss(const V & p, Cache & ch, double kinetic)
+{
+        if constexpr (V::coarse_index)
+                return cached_index_miss(p.K, p.n_energies, kinetic, ch.k, &p);
+        else
+                return cached_index_miss(p.K, p.n_energies, kinetic, ch.k);
 }
 
 /* src/pumas.c:4146-4157. */
@@ -366,7 +442,11 @@ PG_FN double interpolate(const V & p, int * last, A tx, B ty, C tm, double x)
         double y0 = ty[i1], y1 = ty[i1 + 1];
         double d0 = tm[i1], d1 = tm[i1 + 1];
         if (!((x >= x0) && (x < x1))) {
-                i1 = cached_index_miss(tx, p.n_energies, x, last);
+                if constexpr (V::coarse_index)
+                        i1 = cached_index_miss(tx, p.n_energies, x, last,
+                            (tx == p.K) ? &p : nullptr);
+                else
+                        i1 = cached_index_miss(tx, p.n_energies, x, last);
                 x0 = tx[i1];
                 x1 = tx[i1 + 1];
                 y0 = ty[i1];
@@ -601,7 +681,7 @@ PG_FN void table_get_msc(const V & p, Cache & ch, int scheme, int material,
                 double a0 = y0[i], a1 = y0[i + 1], b0 = m0[i], b1 = m0[i + 1];
                 double c0 = y1[i], c1 = y1[i + 1], e0 = m1[i], e1 = m1[i + 1];
                 if (!((kinetic >= x0) && (kinetic < x1))) {
-                        i = cached_index_miss(p.K, p.n_energies, kinetic, ch.k);
+                        i = k_miss(p, ch, kinetic);
                         x0 = p.K[i];
                         x1 = p.K[i + 1];
                         a0 = y0[i];
@@ -636,7 +716,7 @@ PG_FN double coulomb_ehs_length(const V & p, Cache & ch, int material, double ki
                 double x0 = p.K[i], x1 = p.K[i + 1];
                 double l0 = Lb[i], l1 = Lb[i + 1];
                 if (!((kinetic >= x0) && (kinetic < x1))) {
-                        i = cached_index_miss(p.K, p.n_energies, kinetic, ch.k);
+                        i = k_miss(p, ch, kinetic);
                         x0 = p.K[i];
                         x1 = p.K[i + 1];
                         l0 = Lb[i];

@@ -49,6 +49,10 @@ struct LdsDoubles {
         {
                 return LdsDoubles{ off + (uint32_t)k };
         }
+        PG_FN bool operator==(const LdsDoubles & o) const
+        {
+                return off == o.off;
+        }
 };
 
 /* The members of PhysicsView that the per-step lookups read (CEL range,
@@ -59,6 +63,11 @@ struct LdsStepTables {
         double mass;
         LdsDoubles K, X, X_dK, K_dX, dE, dE_dK, Omega, Omega_dK;
         LdsDoubles CS, CS_dK, Mu0, Mu0_dK, Ms1, Ms1_dK, Lb, a_max, b_max;
+        const int16_t * k_coarse; /* in HBM, as PhysicsView::k_coarse */
+        int k_bin0, k_nbins;
+        /* The stepping kernel's lookups rarely miss far: no coarse index
+         * code in it. */
+        static constexpr bool coarse_index = false;
 };
 
 /* ------------------------------------------------------------------------
@@ -247,13 +256,27 @@ extern "C" enum pumas_return pumas_gpu_physics_upload_memory(
                 while (step.size() % 8) step.push_back(0.);
         }
 
+        /* Coarse index of the kinetic energy grid (k_index_fast). */
+        std::vector<int16_t> coarse(4096);
+        int k_bin0 = 0;
+        {
+                const Section * sk = find(sections, "K");
+                const double * Kh =
+                    reinterpret_cast<const double *>(image + sk->offset);
+                const int nb = k_coarse_build(Kh, (int)sk->count,
+                    coarse.data(), (int)coarse.size(), &k_bin0);
+                coarse.resize(nb);
+        }
+
         /* Device image: the PTAB file as is, then the derived sections. */
         const size_t extra_offset = (size + 255) / 256 * 256;
         const size_t float_offset = (extra_offset +
             component0.size() * sizeof(int32_t) + 255) / 256 * 256;
         const size_t step_offset = (float_offset +
             floats.size() * sizeof(float) + 255) / 256 * 256;
-        const size_t total = step_offset + step.size() * sizeof(double);
+        const size_t coarse_offset = (step_offset +
+            step.size() * sizeof(double) + 255) / 256 * 256;
+        const size_t total = coarse_offset + coarse.size() * sizeof(int16_t);
         void * device = nullptr;
         HIP_TRY(hipMalloc(&device, total));
         hipError_t e1 = hipMemcpy(device, image, size, hipMemcpyHostToDevice);
@@ -265,8 +288,11 @@ extern "C" enum pumas_return pumas_gpu_physics_upload_memory(
             hipMemcpyHostToDevice);
         hipError_t e4 = hipMemcpy((uint8_t *)device + step_offset,
             step.data(), step.size() * sizeof(double), hipMemcpyHostToDevice);
+        hipError_t e5 = coarse.empty() ? hipSuccess :
+            hipMemcpy((uint8_t *)device + coarse_offset, coarse.data(),
+                coarse.size() * sizeof(int16_t), hipMemcpyHostToDevice);
         if ((e1 != hipSuccess) || (e2 != hipSuccess) || (e3 != hipSuccess) ||
-            (e4 != hipSuccess)) {
+            (e4 != hipSuccess) || (e5 != hipSuccess)) {
                 (void)hipFree(device);
                 return pg_fail(PUMAS_RETURN_INTERNAL_ERROR,
                     "could not copy the tables to the device");
@@ -351,6 +377,10 @@ extern "C" enum pumas_return pumas_gpu_physics_upload_memory(
         I32(component_element, "component_element");
         I32(element_index, "element_index");
         v.material_component0 = (const int *)(base + extra_offset);
+        v.k_coarse = coarse.empty() ? nullptr :
+            (const int16_t *)(base + coarse_offset);
+        v.k_bin0 = k_bin0;
+        v.k_nbins = (int)coarse.size();
 
         FloatTables & ft = ph->floats;
         {
@@ -396,6 +426,9 @@ extern "C" enum pumas_return pumas_gpu_physics_upload_memory(
                 st.n_energies = v.n_energies;
                 st.n_materials = v.n_materials;
                 st.mass = v.mass;
+                st.k_coarse = v.k_coarse;
+                st.k_bin0 = v.k_bin0;
+                st.k_nbins = v.k_nbins;
                 ph->step = (const double *)((const uint8_t *)device +
                     step_offset);
                 ph->n_step = (uint32_t)step.size();
@@ -412,6 +445,9 @@ extern "C" enum pumas_return pumas_gpu_physics_upload_memory(
                 rt.n_energies = st.n_energies;
                 rt.n_materials = st.n_materials;
                 rt.mass = st.mass;
+                rt.k_coarse = st.k_coarse;
+                rt.k_bin0 = st.k_bin0;
+                rt.k_nbins = st.k_nbins;
                 rt.K.off = base + (uint32_t)step_at[0];
                 rt.X.off = base + (uint32_t)step_at[1];
                 rt.X_dK.off = base + (uint32_t)step_at[2];

@@ -20,6 +20,7 @@ static std::vector<uint8_t> g_image;
 static PhysicsView g_view;
 static ModelView g_mv;
 static std::vector<int> g_c0;
+static std::vector<int16_t> g_coarse;
 
 static const void *sec(const char *name) {
     const ptab_header *h = (const ptab_header *)g_image.data();
@@ -55,6 +56,9 @@ extern "C" int host_load(const char *path) {
     g_c0.resize(v.n_materials); int acc = 0;
     for (int m = 0; m < v.n_materials; m++) { g_c0[m] = acc; acc += v.elements_in[m]; }
     v.material_component0 = g_c0.data();
+    g_coarse.assign(4096, 0);
+    v.k_nbins = k_coarse_build(v.K, v.n_energies, g_coarse.data(), (int)g_coarse.size(), &v.k_bin0);
+    v.k_coarse = v.k_nbins ? g_coarse.data() : nullptr;
     g_mv.z13 = (const double*)sec("element_z13"); g_mv.zm13 = (const double*)sec("element_zm13");
     g_mv.z23 = (const double*)sec("element_z23"); g_mv.dn = (const double*)sec("element_dn");
     g_mv.lrad = v.element_lrad; g_mv.cst = (const double*)sec("constants");
@@ -104,4 +108,34 @@ extern "C" int host_transport(const pumas_gpu_context *ctx, const pumas_gpu_geom
         events[i] = w.event; media[2*i] = w.media0; media[2*i+1] = w.end_medium; steps[i] = t.steps; draws[i] = t.rng.count; rcs[i] = w.rc;
     }
     return 0;
+}
+
+/* Number of values in [K[0], K[n - 1]) whose coarse index row differs from
+ * the bisection's: every grid value, its neighbours in double precision,
+ * and `n_random` log-uniform values (a fixed LCG). */
+extern "C" long host_check_k_index(long n_random) {
+    const PhysicsView &p = g_view;
+    const int n = p.n_energies;
+    long bad = 0;
+    auto check = [&](double x) {
+        if (!(x >= p.K[0]) || !(x < p.K[n - 1])) return;
+        const int a = k_index_fast(p, x);
+        const int b = table_index(p.K, n, x);
+        if ((a >= 0) && (a != b)) bad++;
+        if ((a < 0) && (x >= p.K[1])) bad++; /* the index must cover the grid */
+    };
+    for (int i = 1; i < n; i++) {
+        check(p.K[i]);
+        check(nextafter(p.K[i], 0.));
+        check(nextafter(p.K[i], 1e300));
+        if (i + 1 < n) check(0.5 * (p.K[i] + p.K[i + 1]));
+    }
+    unsigned long long s = 12345;
+    const double l0 = log(p.K[1]), l1 = log(p.K[n - 1]);
+    for (long k = 0; k < n_random; k++) {
+        s = s * 6364136223846793005ull + 1442695040888963407ull;
+        const double u = (double)(s >> 11) / 9007199254740992.0;
+        check(exp(l0 + u * (l1 - l0)));
+    }
+    return bad;
 }

@@ -178,3 +178,15 @@ def test_non_finite_energy_ends_with_value_error(host, direction, energy):
         assert rc[i] == 0
         assert st[i] == ok[3][i]
         assert s["energy"][i] == ok[0]["energy"][i]
+
+
+@pytest.mark.parametrize("physics", ["standard_c005", "iron_c001"])
+def test_coarse_kinetic_index_matches_bisection(host, physics):
+    """The O(1) bracket of the kinetic energy grid (exponent and leading
+    mantissa bits, then a few comparisons) is the bisection's row for every
+    grid value, its double precision neighbours, the mid points and 10^6
+    log-uniform values."""
+    host.host_check_k_index.restype = ctypes.c_long
+    host.host_check_k_index.argtypes = [ctypes.c_long]
+    assert host.host_load(pa.physics_path(physics).encode()) == 0
+    assert host.host_check_k_index(1_000_000) == 0
