@@ -1592,12 +1592,14 @@ struct EngineT {
                 const double kinetic = t.energy;
                 double mu0 = 0., invlb1 = 0.;
                 /* The FP64 step tables (st) hold the MSC columns in FP64
-                 * mode; the FP32-bulk step view holds the range columns
-                 * only. */
-                if constexpr (F32)
-                        table_get_msc(p, t.ix, c.energy_loss, material,
-                            kinetic, &mu0, &invlb1);
-                else
+                 * mode; FP32-bulk reads its single precision mirrors. */
+                if constexpr (F32) {
+                        float m, l;
+                        ftable_get_msc(ft, t.fx, c.energy_loss, material,
+                            (float)kinetic, &m, &l);
+                        mu0 = m;
+                        invlb1 = l;
+                } else
                         table_get_msc(st, t.ix, c.energy_loss, material,
                             kinetic, &mu0, &invlb1);
 
