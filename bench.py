@@ -81,11 +81,11 @@ def parse():
                          "workloads take a few thousand)")
     ap.add_argument("--tail-del", type=int, default=0,
                     help="DELs inside the stepping wavefronts once the step "
-                         "queue is empty: 1 always, -1 never, 0 below 2^22 "
-                         "histories")
+                         "queue is empty: 1 always, -1 never, 0 in the rounds "
+                         "that fit the resident lanes")
     ap.add_argument("--lds-tables", type=int, default=0,
-                    help="FP32 tables of the stepping kernel in LDS: 0 when "
-                         "they fit, -1 never")
+                    help="step tables of the stepping kernel in LDS: 0 "
+                         "when they fit, -1 never")
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--serve-seconds", type=float, default=0.25,
