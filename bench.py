@@ -70,7 +70,7 @@ def parse():
                          "per-launch timing of the stepping kernel then)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (default launch settings)")
-    ap.add_argument("--tune-n", type=int, default=2_000_000,
+    ap.add_argument("--tune-n", type=int, default=4_000_000,
                     help="histories of the autotuner's sample")
     ap.add_argument("--lines", type=int, default=4096,
                     help="lines of sight of the terrain workload (10^3 to "
