@@ -299,6 +299,26 @@ enum pumas_return pumas_gpu_autotune(const pumas_gpu_physics * physics,
     size_t n, uint64_t seed, enum pumas_gpu_precision precision,
     struct pumas_gpu_options * best, double * steps_per_s, void * stream);
 
+/*
+ * Per-history status codes (outputs->rc) beyond enum pumas_return, for
+ * histories whose end state is not the reference's. They are not errors of
+ * the call (pumas_gpu_transport still returns the first error code of
+ * enum pumas_return, or success), they are counted in pumas_gpu_stats, and
+ * such a history ends with event PUMAS_EVENT_NONE (a tally with an event
+ * mask leaves it out).
+ */
+enum pumas_gpu_history_status {
+        /* options->max_steps Monte Carlo steps were taken (the reference has
+         * no cap). */
+        PUMAS_GPU_HISTORY_STEP_LIMIT = 100,
+        /* A rejection or bisection loop reached 65536 iterations (from a
+         * non finite input: the reference would not return). */
+        PUMAS_GPU_HISTORY_LOOP_LIMIT = 101,
+        /* The target selection of a discrete energy loss matched no target
+         * (the reference aborts on assert(0), src/pumas.c:6207, 6264). */
+        PUMAS_GPU_HISTORY_NO_TARGET = 102
+};
+
 /* Optional per-history outputs (device pointers, each may be NULL). */
 struct pumas_gpu_outputs {
         enum pumas_event * events;  /* [n] end condition, as `*event` */
@@ -319,6 +339,11 @@ struct pumas_gpu_stats {
         uint64_t rounds;        /* host checks of the queue lengths */
         uint64_t lds_bytes;     /* LDS copy of the step tables per block of
                                    the stepping kernel (0: read from HBM) */
+        /* Histories ended with a status of enum pumas_gpu_history_status
+         * (their end state is not the reference's). */
+        uint64_t step_limited;  /* PUMAS_GPU_HISTORY_STEP_LIMIT */
+        uint64_t loop_limited;  /* PUMAS_GPU_HISTORY_LOOP_LIMIT */
+        uint64_t no_target;     /* PUMAS_GPU_HISTORY_NO_TARGET */
 };
 
 /*

@@ -18,7 +18,6 @@ from pumas_amd._abi import STATE_DTYPE
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 LIBRARY = os.path.join(HERE, "_ref", "libpumas_ref.so")
-LIBRARY_ALIGNED = os.path.join(HERE, "_ref", "libpumas_ref_aligned.so")
 MDF = {
     "standard_c005": (os.path.join(REPO, "pumas_amd", "export",
                                    "materials_standard.xml"), 0.05),
@@ -29,15 +28,14 @@ MDF = {
 _libs = {}
 
 
-def available(aligned=False):
-    return os.path.exists(LIBRARY_ALIGNED if aligned else LIBRARY)
+def available():
+    return os.path.exists(LIBRARY)
 
 
-def library(aligned=False):
-    """The oracle library; `aligned` selects the build whose float logf /
-    expf are rounded double precision evaluations."""
-    if aligned not in _libs:
-        path = LIBRARY_ALIGNED if aligned else LIBRARY
+def library():
+    """The oracle library."""
+    if "ref" not in _libs:
+        path = LIBRARY
         if not os.path.exists(path):
             raise FileNotFoundError(
                 f"{path} is missing; build it with `make -C oracle`")
@@ -77,19 +75,18 @@ def library(aligned=False):
         lib.ref_dcs_exact.restype = ctypes.c_double
         lib.ref_dilog.argtypes = [ctypes.c_double]
         lib.ref_dilog.restype = ctypes.c_double
-        _libs[aligned] = lib
-    return _libs[aligned]
+        _libs["ref"] = lib
+    return _libs["ref"]
 
 
 class RefPhysics:
     """Reference physics created by pumas_physics_create from the MDF the
     PTAB file of the same name was exported from."""
 
-    def __init__(self, name, aligned=False):
+    def __init__(self, name):
         mdf, cutoff = MDF[name]
         self.name = name
-        self.aligned = aligned
-        self.lib = library(aligned)
+        self.lib = library()
         self._dedx = tempfile.mkdtemp(prefix="pumas_dedx_")
         self.handle = ctypes.c_void_p()
         rc = self.lib.ref_physics_create(mdf.encode(), self._dedx.encode(),
@@ -125,9 +122,9 @@ class RefPhysics:
 
 
 class RefGeometry:
-    def __init__(self, description, aligned=False):
+    def __init__(self, description):
         self.description = description
-        self.lib = library(aligned)
+        self.lib = library()
         self._desc = description.to_c()
         self.handle = self.lib.ref_geometry_create(ctypes.byref(self._desc))
 
@@ -212,7 +209,7 @@ def transport_record(physics, context, geometry, states, seed=0,
     """Transport on one thread with the reference's recorder (period as
     `pumas_recorder.period`); returns (final states, frames) with frames a
     dict of arrays: states, media (index, -1 = outside), events, history."""
-    lib = library(physics.aligned)
+    lib = library()
     f = lib.ref_transport_record
     f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                   ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64,

@@ -18,13 +18,13 @@ import os
 import numpy as np
 
 from . import _abi
-from ._abi import (Event, Flux, GeometryKind, Mode, Precision, Return, Step,
-                   STATE_DTYPE)
+from ._abi import (Event, Flux, GeometryKind, HistoryStatus, Mode, Precision,
+                   Return, Step, STATE_DTYPE)
 
 __all__ = [
     "Context", "Event", "Flux", "Geometry", "GeometryDescription",
     "Recorder",
-    "GeometryKind", "Mode", "Physics", "Precision", "PumasError", "Return",
+    "GeometryKind", "HistoryStatus", "Mode", "Physics", "Precision", "PumasError", "Return",
     "STATE_DTYPE", "Server", "Step", "Tally", "TallySpec", "library",
     "new_states",
     "autotune", "physics_path", "tally", "tally_device", "transport",
@@ -271,6 +271,10 @@ class TransportResult:
     total_steps: int = 0
     kernel_ms: float = 0.0
     lds_bytes: int = 0
+    # Histories ended with a status code (enum pumas_gpu_history_status).
+    step_limited: int = 0
+    loop_limited: int = 0
+    no_target: int = 0
 
 
 @dataclasses.dataclass
@@ -283,12 +287,16 @@ class TransportStats:
     step_launches: int = 0
     rounds: int = 0
     lds_bytes: int = 0
+    step_limited: int = 0
+    loop_limited: int = 0
+    no_target: int = 0
 
     @classmethod
     def from_c(cls, st):
         return cls(int(st.steps), int(st.histories), float(st.kernel_ms),
                    float(st.step_kernel_ms), int(st.step_launches),
-                   int(st.rounds), int(st.lds_bytes))
+                   int(st.rounds), int(st.lds_bytes), int(st.step_limited),
+                   int(st.loop_limited), int(st.no_target))
 
 
 def _options(precision, block_size, blocks_per_cu, max_steps,
@@ -354,6 +362,9 @@ def transport(physics, context, geometry, states, seed=0, first_history=0,
     res.total_steps = int(stats.steps)
     res.kernel_ms = float(stats.kernel_ms)
     res.lds_bytes = int(stats.lds_bytes)
+    res.step_limited = int(stats.step_limited)
+    res.loop_limited = int(stats.loop_limited)
+    res.no_target = int(stats.no_target)
     return res
 
 

@@ -74,6 +74,14 @@ class Return(enum.IntEnum):
     VALUE_ERROR = 24
 
 
+class HistoryStatus(enum.IntEnum):
+    """enum pumas_gpu_history_status: per-history codes (outputs rc) of
+    histories whose end state is not the reference's."""
+    STEP_LIMIT = 100
+    LOOP_LIMIT = 101
+    NO_TARGET = 102
+
+
 class GeometryKind(enum.IntEnum):
     INFINITE = 0
     SLABS = 1
@@ -157,7 +165,10 @@ class GpuStats(ctypes.Structure):
                 ("kernel_ms", ctypes.c_double),
                 ("step_kernel_ms", ctypes.c_double),
                 ("step_launches", ctypes.c_uint64),
-                ("rounds", ctypes.c_uint64), ("lds_bytes", ctypes.c_uint64)]
+                ("rounds", ctypes.c_uint64), ("lds_bytes", ctypes.c_uint64),
+                ("step_limited", ctypes.c_uint64),
+                ("loop_limited", ctypes.c_uint64),
+                ("no_target", ctypes.c_uint64)]
 
 
 # Every entry point declared by include/pumas_gpu.h.

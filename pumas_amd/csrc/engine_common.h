@@ -15,6 +15,8 @@
 
 #include <stdint.h>
 
+#include "libm.h"
+
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define PG_FN __device__ __forceinline__
@@ -91,6 +93,16 @@ enum {
         RETURN_MEDIUM_ERROR = 12,
         RETURN_MISSING_LIMIT = 15,
         RETURN_VALUE_ERROR = 24
+};
+
+/* Per-history status of a history whose end state is not the reference's
+ * (enum pumas_gpu_history_status of include/pumas_gpu.h). Not errors of
+ * the call: they are counted, and the history's rc carries them. */
+enum {
+        HISTORY_STEP_LIMIT = 100, /* the step cap ended it */
+        HISTORY_LOOP_LIMIT = 101, /* a sampling or bisection loop gave up */
+        HISTORY_NO_TARGET = 102   /* backward DEL target selection found no
+                                     target: the reference aborts */
 };
 
 enum { STEP_CHECK = 0, STEP_RAW = 1 };
@@ -182,15 +194,6 @@ struct Rng {
          * so that both engines consume one random sequence. */
         PG_FN float uniformf() { return (float)uniform(); }
 };
-
-/* ------------------------------------------------------------------------
- * Single precision logarithm and exponential used by the FP32 DCS spline.
- * The reference calls the C library logf / expf (src/pumas.c:11298-11346);
- * here the value is computed in double precision and rounded once, i.e. a
- * correctly rounded result except for rare double rounding cases.
- * ------------------------------------------------------------------------ */
-PG_FN float logf_cr(float x) { return (float)log((double)x); }
-PG_FN float expf_cr(float x) { return (float)exp((double)x); }
 
 /* ------------------------------------------------------------------------
  * Physics tables in device memory (PTAB sections, see ptab.h).
@@ -473,7 +476,7 @@ PG_FN double cel_grammage(const V & p, Cache & ch, int scheme, int material, dou
                 const double K0 = p.K[imax];
                 const double K1 = a / b + p.mass;
                 return *tab2(p, p.X, scheme, material, imax) +
-                    1. / b * log((kinetic + K1) / (K0 + K1));
+                    1. / b * lm::log((kinetic + K1) / (K0 + K1));
         }
         return interpolate(p, ch.k, p.K, tab2(p, p.X, scheme, material, 0),
             tab2(p, p.X_dK, scheme, material, 0), kinetic);
@@ -492,11 +495,11 @@ PG_FN double cel_grammage_as_time(const V & p, Cache & ch, int scheme, int mater
                 const double E0 = p.K[imax] + p.mass;
                 const double tmax = T[imax];
                 const double r =
-                    E0 / (a + b * E0) * exp(a * (time - tmax) / p.mass);
+                    E0 / (a + b * E0) * lm::exp(a * (time - tmax) / p.mass);
                 const double kinetic = a * r / (1. - b * r) - p.mass;
                 const double K1 = a / b + p.mass;
                 return *tab2(p, p.X, scheme, material, imax) +
-                    1. / b * log((kinetic + K1) / (E0 - p.mass + K1));
+                    1. / b * lm::log((kinetic + K1) / (E0 - p.mass + K1));
         }
         return interpolate(p, ch.x, T, tab2(p, p.X, scheme, material, 0),
             tab2(p, p.X_dT, scheme, material, 0), time);
@@ -514,7 +517,7 @@ PG_FN double cel_proper_time(const V & p, Cache & ch, int scheme, int material, 
                 const double E0 = p.K[imax] + p.mass;
                 const double E1 = kinetic + p.mass;
                 return *tab2(p, p.T, scheme, material, imax) +
-                    p.mass / a * log((E1 / E0) * (a + b * E0) / (a + b * E1));
+                    p.mass / a * lm::log((E1 / E0) * (a + b * E0) / (a + b * E1));
         }
         return interpolate(p, ch.k, p.K, tab2(p, p.T, scheme, material, 0),
             tab2(p, p.T_dK, scheme, material, 0), kinetic);
@@ -532,7 +535,7 @@ PG_FN double cel_kinetic_energy(const V & p, Cache & ch, int scheme, int materia
                 const double b = b_max(p, scheme, material);
                 const double K0 = p.K[imax];
                 const double K1 = a / b + p.mass;
-                return (K0 + K1) * exp(b * (grammage - X[imax])) - K1;
+                return (K0 + K1) * lm::exp(b * (grammage - X[imax])) - K1;
         }
         return interpolate(
             p, ch.x, X, p.K, tab2(p, p.K_dX, scheme, material, 0), grammage);
@@ -590,7 +593,7 @@ PG_FN double del_interaction_length(const V & p, Cache & ch, int material, doubl
                 const double b0 = b_max(p, MODE_MIXED, material);
                 const double cs = *tab1(p, p.CS, material, imax);
                 const double dZ = cs / b0 *
-                    log((a0 + b0 * (kinetic + p.mass)) /
+                    lm::log((a0 + b0 * (kinetic + p.mass)) /
                         (a0 + b0 * (k0 + p.mass)));
                 return *tab1(p, p.NI_in, material, imax) + dZ;
         }
@@ -611,7 +614,7 @@ PG_FN double del_kinetic_from_interaction_length(const V & p, Cache & ch, int ma
                 const double b0 = b_max(p, MODE_MIXED, material);
                 const double cs = *tab1(p, p.CS, material, imax);
                 const double nI0 = NI[imax];
-                return ((a0 + b0 * (k0 + p.mass)) * exp(b0 * (nI - nI0) / cs) -
+                return ((a0 + b0 * (k0 + p.mass)) * lm::exp(b0 * (nI - nI0) / cs) -
                            a0) / b0 - p.mass;
         }
         return interpolate(

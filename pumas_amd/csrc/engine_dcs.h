@@ -12,10 +12,10 @@ namespace pg {
 
 /* Extra per physics data used by the exact DCS models. */
 struct ModelView {
-        const double * z13;   /* pow(Z, 1/3) per element */
-        const double * zm13;  /* pow(Z, -1/3) */
-        const double * z23;   /* pow(Z, 2/3) */
-        const double * dn;    /* 1.54 pow(A, 0.27) */
+        const double * z13;   /* lm::pow(Z, 1/3) per element */
+        const double * zm13;  /* lm::pow(Z, -1/3) */
+        const double * z23;   /* lm::pow(Z, 2/3) */
+        const double * dn;    /* 1.54 lm::pow(A, 0.27) */
         const double * lrad;  /* radiation logarithm */
         const double * cst;   /* e, e^0.5, e^-1/6, e^1/3, e^-1/3, (alpha re)^2 */
         const double *gq9_x, *gq9_w, *gq12_x, *gq12_w;
@@ -62,23 +62,23 @@ PG_FN double dcs_bremsstrahlung_SSR(const ModelView & mv, int iel, double Z,
         const double mu_qc = m / (SSR_MMU * mv.cst[0] / Dn);
         const double rho = sqrt(1. + 4. * mu_qc * mu_qc);
 
-        const double log_rho = log((rho + 1.) / (rho - 1.));
-        const double delta1 = log(mu_qc) + 0.5 * rho * log_rho;
-        const double delta2 = log(mu_qc) +
+        const double log_rho = lm::log((rho + 1.) / (rho - 1.));
+        const double delta1 = lm::log(mu_qc) + 0.5 * rho * log_rho;
+        const double delta2 = lm::log(mu_qc) +
             0.25 * (3. * rho - rho * rho * rho) * log_rho + 2. * mu_qc * mu_qc;
 
         const double delta = m * m * v / (2. * energy * (1. - v));
 
-        double phi1 = log(rad_log * Z13 * (m / SSR_ME) /
+        double phi1 = lm::log(rad_log * Z13 * (m / SSR_ME) /
             (1. + rad_log * Z13 * mv.cst[1] * delta / SSR_ME));
-        double phi2 = log(rad_log * Z13 * mv.cst[2] * (m / SSR_ME) /
+        double phi2 = lm::log(rad_log * Z13 * mv.cst[2] * (m / SSR_ME) /
             (1. + rad_log * Z13 * mv.cst[3] * delta / SSR_ME));
         phi1 -= delta1 * (1. - 1. / Z);
         phi2 -= delta2 * (1. - 1. / Z);
 
-        const double s_atomic_1 = log(m / delta /
+        const double s_atomic_1 = lm::log(m / delta /
             (m * delta / (SSR_ME * SSR_ME) + SSR_SQRTE));
-        const double s_atomic_2 = log(1. + SSR_ME /
+        const double s_atomic_2 = lm::log(1. + SSR_ME /
             (delta * rad_log_inel * Z13 * Z13 * SSR_SQRTE));
         const double s_atomic = (4. / 3. * (1. - v) + v * v) *
             (s_atomic_1 - s_atomic_2);
@@ -92,12 +92,12 @@ PG_FN double dcs_bremsstrahlung_SSR(const ModelView & mv, int iel, double Z,
                 s_rad = b0 + b1 * v + b2 * v * v + b3 * v * v * v;
         } else if (v >= 0.01 && v < 0.9) {
                 s_rad = c0 + c1 * v + c2 * v * v;
-                const double tmp = log(1. - v);
-                s_rad += c3 * v * log(v) + c4 * tmp + c5 * tmp * tmp;
+                const double tmp = lm::log(1. - v);
+                s_rad += c3 * v * lm::log(v) + c4 * tmp + c5 * tmp * tmp;
         } else {
                 s_rad = d0 + d1 * v + d2 * v * v;
-                const double tmp = log(1. - v);
-                s_rad += d3 * v * log(v) + d4 * tmp + d5 * tmp * tmp;
+                const double tmp = lm::log(1. - v);
+                s_rad += d3 * v * lm::log(v) + d4 * tmp + d5 * tmp * tmp;
         }
 
         const double result = ((2. - 2. * v + v * v) * phi1 - 2. / 3. *
@@ -144,12 +144,12 @@ PG_FN double dilog(double x)
         if (T <= -0.5) {
                 Y = -(1. + T) / T;
                 S = 1.;
-                A = log(-T);
-                A = -PI6 + A * (-0.5 * A + log(1. + T));
+                A = lm::log(-T);
+                A = -PI6 + A * (-0.5 * A + lm::log(1. + T));
         } else {
                 Y = -T / (1. + T);
                 S = -1.;
-                const double l = log(1 + T);
+                const double l = lm::log(1 + T);
                 A = 0.5 * (l * l);
         }
         const double H = Y + Y - 1;
@@ -198,9 +198,9 @@ PG_FN double dcs_pair_production_d2_SSR(const ModelView & mv, int iel,
                 g2 = 5.3E-05;
         }
 
-        const double zeta1 = (0.073 * log(energy / m /
+        const double zeta1 = (0.073 * lm::log(energy / m /
             (1. + g1 * mv.z23[iel] * energy / m)) - 0.26);
-        const double zeta2 = (0.058 * log(energy / m /
+        const double zeta2 = (0.058 * lm::log(energy / m /
             (1 + g2 / Z13 * energy / m)) - 0.14);
 
         double zeta;
@@ -218,45 +218,45 @@ PG_FN double dcs_pair_production_d2_SSR(const ModelView & mv, int iel,
 
         /* Diagram e */
         const double Be = ((2. + rho2) * (1. + beta) +
-            xi * (3. + rho2)) * log(1. + 1. / xi) +
+            xi * (3. + rho2)) * lm::log(1. + 1. / xi) +
             (1. - rho2 - beta) / (1. + xi) - (3. + rho2);
 
         const double Ce2 = ((1. - rho2) * (1. + beta) +
-            xi * (3. - rho2)) * log(1. + 1. / xi) +
+            xi * (3. - rho2)) * lm::log(1. + 1. / xi) +
             2. * (1. - beta - rho2) / (1. + xi) - (3. - rho2);
         const double Ce1 = Be - Ce2;
 
         const double De = ((2. + rho2) * (1. + beta) +
             xi * (3. + rho2)) * dilog(1. / (1. + xi)) -
-            (2. + rho2) * xi * log(1. + 1. / xi) -
+            (2. + rho2) * xi * lm::log(1. + 1. / xi) -
             (xi + rho2 + beta) / (1. + xi);
 
         double Le1, Le2;
         if (De / Be > 0.) {
                 const double tmp = De / Be;
-                const double Xe = (tmp < 100) ? exp(-tmp) : 0.;
-                Le1 = log(rad_log * Z13 * sqrt(1. + xi) /
+                const double Xe = (tmp < 100) ? lm::exp(-tmp) : 0.;
+                Le1 = lm::log(rad_log * Z13 * sqrt(1. + xi) /
                     (Xe + 2. * SSR_ME * e05 * rad_log * Z13 * (1. + xi) /
                     (energy * v * (1. - rho2)))) - tmp -
-                    0.5 * log(Xe + sq(SSR_ME / m * d_n) * (1. + xi));
+                    0.5 * lm::log(Xe + sq(SSR_ME / m * d_n) * (1. + xi));
 
-                Le2 = log(rad_log * Z13 * e16 * sqrt(1 + xi) /
+                Le2 = lm::log(rad_log * Z13 * e16 * sqrt(1 + xi) /
                     (Xe + 2. * SSR_ME * e13 * rad_log * Z13 * (1. + xi) /
                     (energy * v * (1. - rho2)))) - tmp -
-                    0.5 * log(Xe + sq(SSR_ME / m * d_n) *
+                    0.5 * lm::log(Xe + sq(SSR_ME / m * d_n) *
                     em13 * (1. + xi));
         } else {
                 const double tmp = De / Be;
-                const double Xe_inv = (tmp > -100) ? exp(De / Be) : 0;
-                Le1 = log(rad_log * Z13 * sqrt(1. + xi) /
+                const double Xe_inv = (tmp > -100) ? lm::exp(De / Be) : 0;
+                Le1 = lm::log(rad_log * Z13 * sqrt(1. + xi) /
                     (1. + Xe_inv * 2. * SSR_ME * e05 * rad_log * Z13 *
                     (1. + xi) / (energy * v * (1. - rho2)))) - 0.5 * De / Be -
-                    0.5 * log(1. + Xe_inv * sq(SSR_ME / m * d_n) * (1. + xi));
+                    0.5 * lm::log(1. + Xe_inv * sq(SSR_ME / m * d_n) * (1. + xi));
 
-                Le2 = log(rad_log * Z13 * e16 * sqrt(1 + xi) /
+                Le2 = lm::log(rad_log * Z13 * e16 * sqrt(1 + xi) /
                     (1. + Xe_inv * 2. * SSR_ME * e13 * rad_log * Z13 *
                     (1. + xi) / (energy * v * (1. - rho2)))) - 0.5 * De / Be -
-                    0.5 * log(1. + Xe_inv * sq(SSR_ME / m * d_n) *
+                    0.5 * lm::log(1. + Xe_inv * sq(SSR_ME / m * d_n) *
                     em13 * (1. + xi));
         }
 
@@ -266,12 +266,12 @@ PG_FN double dcs_pair_production_d2_SSR(const ModelView & mv, int iel,
 
         /* Diagram mu */
         const double Bm = ((1. + rho2) * (1. + (3. * beta) / 2) - 1. / xi *
-            (1. + 2. * beta) * (1. - rho2)) * log(1. + xi) +
+            (1. + 2. * beta) * (1. - rho2)) * lm::log(1. + xi) +
             xi * (1. - rho2 - beta) / (1. + xi) +
             (1. + 2. * beta) * (1. - rho2);
 
         const double Cm2 = ((1. - beta) * (1. - rho2) -
-            xi * (1. + rho2)) * log(1. + xi) / xi -
+            xi * (1. + rho2)) * lm::log(1. + xi) / xi -
             2. * (1. - beta - rho2) / (1. + xi) +
             1. - beta - (1. + beta) * rho2;
         const double Cm1 = Bm - Cm2;
@@ -279,26 +279,26 @@ PG_FN double dcs_pair_production_d2_SSR(const ModelView & mv, int iel,
         const double Dm = ((1. + rho2) * (1. + (3. * beta) / 2.) -
             1. / xi * (1. + 2. * beta) * (1. - rho2)) *
             dilog(xi / (1. + xi)) + (1. + (3. * beta) / 2.) *
-            (1. - rho2) / xi * log(1. + xi) + (1. - rho2 - beta / 2. *
+            (1. - rho2) / xi * lm::log(1. + xi) + (1. - rho2 - beta / 2. *
             (1. + rho2) + (1. - rho2) / (2. * xi) * beta) * xi / (1. + xi);
 
         double Lm1, Lm2;
         if (Dm / Bm > 0.) {
                 const double tmp = Dm / Bm;
-                const double Xm = (tmp < 100) ? exp(-tmp) : 0.;
-                Lm1 = log(m / SSR_ME * rad_log * Z13 / d_n /
+                const double Xm = (tmp < 100) ? lm::exp(-tmp) : 0.;
+                Lm1 = lm::log(m / SSR_ME * rad_log * Z13 / d_n /
                     (Xm + 2. * SSR_ME * e05 * rad_log * Z13 * (1. + xi) /
                     (energy * v * (1. - rho2)))) - tmp;
-                Lm2 = log(m / SSR_ME * rad_log * Z13 / d_n /
+                Lm2 = lm::log(m / SSR_ME * rad_log * Z13 / d_n /
                     (Xm + 2. * SSR_ME * e13 * rad_log * Z13 * (1. + xi) /
                     (energy * v * (1. - rho2)))) - tmp;
         } else {
                 const double tmp = Dm / Bm;
-                const double Xmv = (tmp > -100) ? exp(tmp) : 0.;
-                Lm1 = log(m / SSR_ME * rad_log * Z13 / d_n /
+                const double Xmv = (tmp > -100) ? lm::exp(tmp) : 0.;
+                Lm1 = lm::log(m / SSR_ME * rad_log * Z13 / d_n /
                     (1. + 2. * SSR_ME * e05 * rad_log * Z13 * (1. + xi) /
                     (energy * v * (1. - rho2)) * Xmv));
-                Lm2 = log(m / SSR_ME * rad_log * Z13 / d_n /
+                Lm2 = lm::log(m / SSR_ME * rad_log * Z13 / d_n /
                     (1. + 2. * SSR_ME * e13 * rad_log * Z13 * (1. + xi) /
                     (energy * v * (1. - rho2)) * Xmv));
         }
@@ -339,11 +339,11 @@ PG_NOINLINE double dcs_pair_production_SSR(const ModelView & mv, int iel,
 
         const double ri = 1. - rmax;
         if ((ri <= 0.) || (ri >= 1.)) return 0.;
-        const double tmin = log(ri);
+        const double tmin = lm::log(ri);
 
         double I = 0.;
         for (int i = 0; i < 12; i++) {
-                const double rho = exp(mv.gq12_x[i] * tmin);
+                const double rho = lm::exp(mv.gq12_x[i] * tmin);
                 I -= dcs_pair_production_d2_SSR(mv, iel, Z, A_, mass, K, q,
                          rho) *
                     rho * mv.gq12_w[i] * tmin;
@@ -386,20 +386,20 @@ PG_FN double f2p_ALLM97(double x, double Q2)
         const double M = 0.5 * (PROTON_MASS + NEUTRON_MASS);
         const double M2 = M * M;
         const double W2 = M2 + Q2 * (1.0 / x - 1.0);
-        const double t = log(log((Q2 + Q02) / Lambda2) / log(Q02 / Lambda2));
+        const double t = lm::log(lm::log((Q2 + Q02) / Lambda2) / lm::log(Q02 / Lambda2));
         const double xP = (Q2 + mP2) / (Q2 + mP2 + W2 - M2);
         const double xR = (Q2 + mR2) / (Q2 + mR2 + W2 - M2);
         const double cP =
-            cP1 + (cP1 - cP2) * (1.0 / (1.0 + pow(t, cP3)) - 1.0);
+            cP1 + (cP1 - cP2) * (1.0 / (1.0 + lm::pow(t, cP3)) - 1.0);
         const double aP =
-            aP1 + (aP1 - aP2) * (1.0 / (1.0 + pow(t, aP3)) - 1.0);
-        const double bP = bP1 + bP2 * pow(t, bP3);
-        const double cR = cR1 + cR2 * pow(t, cR3);
-        const double aR = aR1 + aR2 * pow(t, aR3);
-        const double bR = bR1 + bR2 * pow(t, bR3);
+            aP1 + (aP1 - aP2) * (1.0 / (1.0 + lm::pow(t, aP3)) - 1.0);
+        const double bP = bP1 + bP2 * lm::pow(t, bP3);
+        const double cR = cR1 + cR2 * lm::pow(t, cR3);
+        const double aR = aR1 + aR2 * lm::pow(t, aR3);
+        const double bR = bR1 + bR2 * lm::pow(t, bR3);
 
-        const double F2P = cP * exp(aP * log(xP) + bP * log(1 - x));
-        const double F2R = cR * exp(aR * log(xR) + bR * log(1 - x));
+        const double F2P = cP * lm::exp(aP * lm::log(xP) + bP * lm::log(1 - x));
+        const double F2R = cR * lm::exp(aR * lm::log(xR) + bR * lm::log(1 - x));
 
         return Q2 / (Q2 + m02) * (F2P + F2R);
 }
@@ -423,9 +423,9 @@ PG_FN double dcs_photonuclear_d2_DRSS(
         if (Z == 1.)
                 shadowing = 1.;
         else if (x < 0.0014)
-                shadowing = exp(-0.1 * log(A));
+                shadowing = lm::exp(-0.1 * lm::log(A));
         else if (x < 0.04)
-                shadowing = exp((0.069 * log10(x) + 0.097) * log(A));
+                shadowing = lm::exp((0.069 * lm::log10(x) + 0.097) * lm::log(A));
         else
                 shadowing = 1.;
         const double F2A = F2p * shadowing *
@@ -464,12 +464,12 @@ PG_NOINLINE double dcs_photonuclear_DRSS(const ModelView & mv, double Z,
 
         /* Logarithmic nodes, as math_gauss_quad_initialise,
          * src/pumas.c:12575-12607. */
-        const double xmin = log(Q2min);
-        const double xmax = log(Q2max);
+        const double xmin = lm::log(Q2min);
+        const double xmax = lm::log(Q2max);
         const double dx = xmax - xmin;
         double ds = 0.;
         for (int i = 0; i < 9; i++) {
-                const double Q2 = exp(xmin + dx * mv.gq9_x[i]);
+                const double Q2 = lm::exp(xmin + dx * mv.gq9_x[i]);
                 const double w = mv.gq9_w[i] * (dx * Q2);
                 double tmp = dcs_photonuclear_d2_DRSS(Z, A, ml, K, q, Q2);
                 if (tmp <= 0.) continue;
@@ -585,7 +585,7 @@ PG_FN double dcs_evaluate(const PhysicsView & p, const ModelView & mv,
         const int nx = p.n_table_dcs;
         const float xmin = p.DCS_x[0];
         const float xmax = p.DCS_x[nx - 1];
-        const float x = logf_cr((float)(q / K));
+        const float x = lm::logf((float)(q / K));
         if ((x < xmin) || (x >= xmax))
                 return dcs_exact(p, mv, ip, iel, K, q) * wj;
         const int ix = bracketf(p.DCS_x, nx, x);
@@ -624,7 +624,7 @@ PG_FN double dcs_evaluate(const PhysicsView & p, const ModelView & mv,
 
                 const float K0 = (float)(p.K[i0]);
                 const float K1 = (float)(p.K[i0 + 1]);
-                h1 = logf_cr(((float)K) / K0) / logf_cr(K1 / K0);
+                h1 = lm::logf(((float)K) / K0) / lm::logf(K1 / K0);
                 h0 = 1.f - h1;
         }
 
@@ -641,13 +641,13 @@ PG_FN double dcs_evaluate(const PhysicsView & p, const ModelView & mv,
         const float t = (x - x0) / dx;
         const float c02 = -3 * (p00 - p01) - 2 * m00 - m01;
         const float c03 = 2 * (p00 - p01) + m00 + m01;
-        float r = expf_cr(p00 + t * (m00 + t * (c02 + t * c03))) * h0;
+        float r = lm::expf(p00 + t * (m00 + t * (c02 + t * c03))) * h0;
         if (h1 > 0.f) {
                 m10 *= dx;
                 m11 *= dx;
                 const float c12 = -3 * (p10 - p11) - 2 * m10 - m11;
                 const float c13 = 2 * (p10 - p11) + m10 + m11;
-                r += expf_cr(p10 + t * (m10 + t * (c12 + t * c13))) * h1;
+                r += lm::expf(p10 + t * (m10 + t * (c12 + t * c13))) * h1;
         }
         return r * wj;
 }
@@ -656,9 +656,9 @@ PG_FN double dcs_evaluate(const PhysicsView & p, const ModelView & mv,
 PG_FN double envelope_norm(double a, double xmin, double xmax)
 {
         if (a == 1) {
-                return log(xmax / xmin);
+                return lm::log(xmax / xmin);
         } else {
-                return (pow(xmin, 1. - a) - pow(xmax, 1. - a)) / (a - 1.);
+                return (lm::pow(xmin, 1. - a) - lm::pow(xmax, 1. - a)) / (a - 1.);
         }
 }
 

@@ -19,7 +19,7 @@ struct PhotonuclearObjective {
 
         PG_FN double operator()(double lnQ2) const
         {
-                const double Q2 = exp(lnQ2);
+                const double Q2 = lm::exp(lnQ2);
                 return -dcs_photonuclear_d2_DRSS(Z, A, m, K, q, Q2) * Q2;
         }
 };
@@ -179,7 +179,7 @@ PG_NOINLINE double polar_bremsstrahlung(const PhysicsView & p, Rng & rng,
                 const double q = muni;
                 const double r = mu0;
                 const double qr = q * r;
-                x0 = (1. + 2 * qr) * log((1. + qr) / (r + qr)) -
+                x0 = (1. + 2 * qr) * lm::log((1. + qr) / (r + qr)) -
                     (1. - r) * (1 + 2 * q) / (1 + q);
         }
 
@@ -203,7 +203,7 @@ PG_NOINLINE double polar_bremsstrahlung(const PhysicsView & p, Rng & rng,
                 const double qr = q * r;
                 if (qr > 1E+05) continue;
 
-                const double x = (1. + 2 * qr) * log((1. + qr) / (r + qr)) -
+                const double x = (1. + 2 * qr) * lm::log((1. + qr) / (r + qr)) -
                     (1. - r) * (1 + 2 * q) / (1 + q);
 
                 pdf *= x / x0;
@@ -228,14 +228,14 @@ PG_NOINLINE double polar_photonuclear(const PhysicsView & p, Rng & rng,
         const PhotonuclearObjective f = { p.element_Z[iel], p.element_A[iel],
                 ml, ki, q };
         double fopt = 0., xopt = 0.;
-        const double lnQ2min = log(Q2min);
-        const double lnQ2max = log(Q2max);
+        const double lnQ2min = lm::log(Q2min);
+        const double lnQ2max = lm::log(Q2max);
         const int status =
             find_minimum(f, lnQ2min, lnQ2max, 1E-06, 100, &xopt, &fopt);
         fopt = -fopt;
         if (status == -2) return 0.;
 
-        const double x = exp(xopt) / (2 * q * M);
+        const double x = lm::exp(xopt) / (2 * q * M);
         const double lne = 0.1;
         if ((x <= 0.04) && (xopt + lne < lnQ2max)) {
                 double f2, x2;
@@ -255,7 +255,7 @@ PG_NOINLINE double polar_photonuclear(const PhysicsView & p, Rng & rng,
         int i;
         for (i = 0; i < 100; i++) {
                 const double u = rng.uniform();
-                Q2 = Q2min * exp(rQ2 * u);
+                Q2 = Q2min * lm::exp(rQ2 * u);
                 const double r =
                     dcs_photonuclear_d2_DRSS(f.Z, f.A, ml, ki, q, Q2) * Q2;
                 if (rng.uniform() * fopt <= r) break;
@@ -361,12 +361,12 @@ PG_FN void coulomb_screening_parameters(const PhysicsView & p, int iel,
         } else {
                 const double r2i = 1. / (1. + zeta2);
                 const double phi = -atan(sqrt(zeta2));
-                f = -0.5 * log(r2i) + 0.5 * (1. - r2i) +
-                    1. / 12. * (1. - cos(2 * phi) * r2i) -
-                    1. / 120. * (1. - cos(4 * phi) * r2i * r2i) +
-                    1. / 252. * (1. - cos(6 * phi) * r2i * r2i * r2i);
+                f = -0.5 * lm::log(r2i) + 0.5 * (1. - r2i) +
+                    1. / 12. * (1. - lm::cos(2 * phi) * r2i) -
+                    1. / 120. * (1. - lm::cos(4 * phi) * r2i * r2i) +
+                    1. / 252. * (1. - lm::cos(6 * phi) * r2i * r2i * r2i);
         }
-        const double ktv = exp(2 * f);
+        const double ktv = lm::exp(2 * f);
 
         const double p02 = kinetic0 * (kinetic0 + 2. * mass);
         const double d = 0.25 * HBAR_C * HBAR_C / p02 * ktv;
@@ -394,7 +394,9 @@ PG_FN double coulomb_nuclear_form_factor(double mu, double N)
                 d = 1. / (1. + 0.1 * x2);
         } else {
                 const double x = sqrt(x2);
-                d = 3 * (sin(x) - x * cos(x)) / (x2 * x);
+                double sx, cx;
+                lm::sincos(x, &sx, &cx);
+                d = 3 * (sx - x * cx) / (x2 * x);
         }
         return d * d;
 }
@@ -468,7 +470,7 @@ PG_FN double coulomb_cs0(double mu, const CoulombData & cd)
         for (int i = 0; i < n; i++) {
                 const double alp = cd.screening[i];
                 const double r = mu / (mu + alp);
-                const double L = log(1. + mu / alp);
+                const double L = lm::log(1. + mu / alp);
                 const double I0 = r / alp;
                 const double J0 = L;
                 const double I1 = L - r;
@@ -478,7 +480,7 @@ PG_FN double coulomb_cs0(double mu, const CoulombData & cd)
 
         const double N = cd.screening[n];
         const double r = mu / (mu + N);
-        const double L = log(1. + mu / N);
+        const double L = lm::log(1. + mu / N);
         const double I0 = r / N;
         const double J0 = L;
         const double I1 = L - r;

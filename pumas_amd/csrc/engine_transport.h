@@ -90,17 +90,23 @@ struct Recorder {
  * away from zero (at most 120 iterations over 10^7 voxel histories), but a
  * NaN or infinite input can make one never accept, and a single such lane
  * would keep its wavefront, and so the launch, from ever ending. After
- * LOOP_MAX iterations a loop gives up: the history is marked as having used
- * all of its steps, so that it ends at its next step boundary as the step
- * cap ends it (EVENT_LIMIT, steps = max_steps), and the loop's last values
- * are used once.
+ * LOOP_MAX iterations a loop gives up: its last values are used once and
+ * the history is marked (max_steps = MARK_LOOP_LIMIT) so that it ends at
+ * its next step boundary with rc HISTORY_LOOP_LIMIT and no event.
+ * Target selections that reach the reference's assert(0) mark the history
+ * the same way (MARK_NO_TARGET, rc HISTORY_NO_TARGET).
  */
-constexpr int LOOP_MAX = 1 << 16;
+#ifndef PG_LOOP_MAX
+#define PG_LOOP_MAX (1 << 16)
+#endif
+constexpr int LOOP_MAX = PG_LOOP_MAX;
+constexpr int MARK_LOOP_LIMIT = -1;
+constexpr int MARK_NO_TARGET = -2;
 
 PG_FN bool loop_stalled(Track & t, int & iterations)
 {
         if (++iterations < LOOP_MAX) return false;
-        t.steps = (uint32_t)t.max_steps;
+        if (t.max_steps >= 0) t.max_steps = MARK_LOOP_LIMIT;
         return true;
 }
 
@@ -236,7 +242,7 @@ struct EngineT {
                 if constexpr (F32)
                         return -(double)logf(t.rng.uniformf());
                 else
-                        return -log(t.rng.uniform());
+                        return -lm::log(t.rng.uniform());
         }
 
         PG_FN double sgn() const
@@ -283,10 +289,10 @@ struct EngineT {
                         t.randn_next = r * cp;
                         return r * s;
                 }
-                const double r = sqrt(-2. * log(t.rng.uniform()));
+                const double r = sqrt(-2. * lm::log(t.rng.uniform()));
                 const double phi = 2. * PI * t.rng.uniform();
-                const double cp = cos(phi);
-                const double s = sin(phi);
+                double s, cp;
+                lm::sincos(phi, &s, &cp);
                 t.randn_next = r * cp;
                 return r * s;
         }
@@ -347,8 +353,7 @@ struct EngineT {
                         sp = sf;
                 } else {
                         const double phi = PI * (1. - 2. * t.rng.uniform());
-                        cp = cos(phi);
-                        sp = sin(phi);
+                        lm::sincos(phi, &sp, &cp);
                 }
                 direction[0] =
                     cos_theta * direction[0] + st * (cp * u0x + sp * u1x);
@@ -482,7 +487,7 @@ struct EngineT {
                             cel_energy_loss(p, t.ix, MODE_CSDA, material, kf) /
                             cel_energy_loss(p, t.ix, MODE_CSDA, material, ki);
                 if (c.decay == MODE_WEIGHTED)
-                        t.weight *= exp(-fabs(ti - t.time) / p.ctau);
+                        t.weight *= lm::exp(-fabs(ti - t.time) / p.ctau);
 
                 const double path =
                     (c.direction == MODE_FORWARD) ? distance : -distance;
@@ -527,7 +532,7 @@ struct EngineT {
                         if (c.scattering == MODE_MIXED) {
                                 const double X = Xi -
                                     coulomb_ehs_length(p, t.ix, material, t.energy) *
-                                        log(t.rng.uniform());
+                                        lm::log(t.rng.uniform());
                                 if ((*grammage_max == 0.) ||
                                     (X < *grammage_max)) {
                                         *grammage_max = X;
@@ -545,7 +550,7 @@ struct EngineT {
                         for (int i = 0; i < 1000; i++) {
                                 const double zeta = t.rng.uniform();
                                 if ((zeta <= 0.) || (zeta >= 1.)) continue;
-                                const double nI = del_interaction_length(p, t.ix, material, t.energy) + s * log(zeta);
+                                const double nI = del_interaction_length(p, t.ix, material, t.energy) + s * lm::log(zeta);
                                 if (nI <= 0.) break;
                                 const double k =
                                     del_kinetic_from_interaction_length(p, t.ix, material, nI);
@@ -566,7 +571,7 @@ struct EngineT {
                 if ((scheme < MODE_STRAGGLED) &&
                     (c.scattering == MODE_MIXED)) {
                         const double nI = ehs_interaction_length(p, t.ix, scheme, material, t.energy) +
-                            s * log(t.rng.uniform());
+                            s * lm::log(t.rng.uniform());
                         if (nI > 0.) {
                                 const double k =
                                     ehs_kinetic_from_interaction_length(p, t.ix, scheme, material, nI);
@@ -1151,9 +1156,11 @@ struct EngineT {
                                                 return;
                                         }
                                 }
-                        /* Not reachable for normalised fractions. */
+                        /* The reference asserts (src/pumas.c:6207); not
+                         * reachable for normalised fractions. */
                         *element = p.component_element[ic0 + nc - 1];
                         *process = N_DEL_PROCESSES - 1;
+                        if (t.max_steps >= 0) t.max_steps = MARK_NO_TARGET;
                         return;
                 } else {
                         const double Q = *weight;
@@ -1206,8 +1213,11 @@ struct EngineT {
                                                 return;
                                         }
                                 }
+                        /* The reference asserts (src/pumas.c:6264): the
+                         * history is marked and ends at its next step. */
                         *element = p.component_element[ic0 + nc - 1];
                         *process = N_DEL_PROCESSES - 1;
+                        if (t.max_steps >= 0) t.max_steps = MARK_NO_TARGET;
                 }
         }
 
@@ -1331,16 +1341,16 @@ struct EngineT {
                                 u = (u - p0) / (1. - p0);
                         }
                         if (a == 1) {
-                                x = xmin * exp(u * log(xmax / xmin));
+                                x = xmin * lm::exp(u * lm::log(xmax / xmin));
                         } else {
                                 const double am = a - 1.;
-                                const double pmin = pow(xmin, -am);
-                                const double pmax = pow(xmax, -am);
-                                x = pow(pmin - u * (pmin - pmax), -1. / am);
+                                const double pmin = lm::pow(xmin, -am);
+                                const double pmax = lm::pow(xmax, -am);
+                                x = lm::pow(pmin - u * (pmin - pmax), -1. / am);
                         }
 
                         const double penv =
-                            b0 * pow(x, -a0) + b1 * pow(x, -a1);
+                            b0 * lm::pow(x, -a0) + b1 * lm::pow(x, -a1);
                         const double d = dcs_evaluate(p, mv, t.ix, process, iel, t.energy, t.energy * x);
                         if (t.rng.uniform() * penv <= d) break;
                 }
@@ -1349,25 +1359,29 @@ struct EngineT {
                 return 1;
         }
 
-        /* del_randomise_power_law, src/pumas.c:6109-6138 (alpha != 1). */
-        PG_FN void del_randomise_power_law(Track & t, double alpha,
-            double xmin, double xmax, double * p_r, double * p_w) const
+        /* del_randomise_power_law, src/pumas.c:6109-6138, for its one use
+         * with alpha = BMC_ALPHA = 2 (src/pumas.c:6066-6068). The exponent
+         * 1 - alpha = -1 is then a constant, and the reference's compiler
+         * rewrites pow(x, -1) as 1 / x (gcc expands powers -1, 0, 1 and 2
+         * exactly); so does this. */
+        PG_FN void del_randomise_power_law(Track & t, double xmin,
+            double xmax, double * p_r, double * p_w) const
         {
+                static_assert(BMC_ALPHA == 2.0, "power law of exponent -1");
                 double r = 0., w_bias;
-                const double a1 = 1. - alpha;
-                const double x0 = pow(xmin, a1);
-                const double x1 = pow(xmax, a1);
+                const double x0 = 1. / xmin;
+                const double x1 = 1. / xmax;
                 int tries = 0;
                 for (;;) {
                         const double z = t.rng.uniform();
                         const double tmp = x0 + z * (x1 - x0);
-                        r = pow(tmp, 1. / a1);
+                        r = 1. / tmp;
                         if ((r < xmin) || (r >= xmax)) {
                                 if (!loop_stalled(t, tries)) continue;
                                 w_bias = 0.;
                                 break;
                         }
-                        w_bias = (x1 - x0) * r / (a1 * tmp);
+                        w_bias = (x1 - x0) * r / (-tmp);
                         break;
                 }
                 *p_r = 1. - r;
@@ -1379,11 +1393,11 @@ struct EngineT {
         PG_NOINLINE int del_randomise_reverse(
             Track & t, int material, int * process_out, int * element) const
         {
-                const double lnq0 = -log(p.cutoff);
+                const double lnq0 = -lm::log(p.cutoff);
                 const double kt = p.Kt[material];
                 const double kf = t.energy;
                 const double pCEL =
-                    (kf >= kt) ? 0. : lnq0 / (lnq0 + log(kt / (kt - kf)));
+                    (kf >= kt) ? 0. : lnq0 / (lnq0 + lm::log(kt / (kt - kf)));
                 if (pCEL && (t.rng.uniform() < pCEL)) {
                         t.weight /= pCEL;
                         return -1;
@@ -1407,7 +1421,7 @@ struct EngineT {
                 }
 
                 double r, w_bias;
-                del_randomise_power_law(t, BMC_ALPHA, xf, xmax, &r, &w_bias);
+                del_randomise_power_law(t, xf, xmax, &r, &w_bias);
                 w_bias /= r;
                 t.energy /= r;
 
@@ -1753,7 +1767,7 @@ struct EngineT {
 
                 if (c.decay == MODE_RANDOMISED) {
                         const double u = t.rng.uniform();
-                        t.lifetime = t.time - p.ctau * log(u);
+                        t.lifetime = t.time - p.ctau * lm::log(u);
                 }
 
                 int do_stepping = 1;
@@ -1860,6 +1874,10 @@ struct EngineT {
                 w.end_medium = w.medium;
                 w.event = t.step_event;
                 w.phase = PHASE_DONE;
+                if ((t.max_steps < 0) && (w.rc == RETURN_SUCCESS))
+                        w.rc = (t.max_steps == MARK_LOOP_LIMIT) ?
+                            HISTORY_LOOP_LIMIT :
+                            HISTORY_NO_TARGET;
                 record_frame(w, w.medium, t.step_event | EVENT_STOP);
         }
 
@@ -1914,8 +1932,11 @@ struct EngineT {
                         return;
                 }
                 if ((int)t.steps >= t.max_steps) {
-                        /* Safety cap: end the history as a limit. */
-                        t.step_event = EVENT_LIMIT;
+                        /* The step cap, or a history marked by a loop that
+                         * gave up or by a failed target selection: it ends
+                         * here with no event and a status code. */
+                        if (t.max_steps >= 0) w.rc = HISTORY_STEP_LIMIT;
+                        t.step_event = EVENT_NONE;
                         walker_finish(w);
                         return;
                 }
@@ -1934,13 +1955,13 @@ struct EngineT {
                 if ((c.direction == MODE_BACKWARD) &&
                     (c.energy_loss >= MODE_CSDA)) {
                         const double wt = (c.decay == MODE_WEIGHTED) ?
-                            exp(-fabs(t.time - w.ti) / p.ctau) :
+                            lm::exp(-fabs(t.time - w.ti) / p.ctau) :
                             1.;
                         t.weight = w.wi * wt * w.dei *
                             cel_energy_loss(p, t.ix, scheme, w.material,
                                 t.energy);
                 } else if (c.decay == MODE_WEIGHTED) {
-                        t.weight = w.wi * exp(-fabs(t.time - w.ti) / p.ctau);
+                        t.weight = w.wi * lm::exp(-fabs(t.time - w.ti) / p.ctau);
                 }
 
                 if (!t.step_event) {

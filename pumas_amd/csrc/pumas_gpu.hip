@@ -709,7 +709,10 @@ enum {
         C_STEP_N0,   /* step queue lengths (ping-pong) */
         C_STEP_N1,
         C_DEL_N,     /* DEL queue length */
-        C_COUNT = 8
+        C_STATUS,    /* histories ended with HISTORY_STEP_LIMIT, then */
+        C_STATUS_LOOP,   /* HISTORY_LOOP_LIMIT and */
+        C_STATUS_TARGET, /* HISTORY_NO_TARGET */
+        C_COUNT = 12
 };
 
 struct KernelArgs {
@@ -814,7 +817,10 @@ __device__ __forceinline__ void walker_store(const KernelArgs & a, uint64_t i,
         if (a.steps) a.steps[i] = t.steps;
         if (a.draws) a.draws[i] = t.rng.count;
         if (a.rc) a.rc[i] = w.rc;
-        if (w.rc != RETURN_SUCCESS)
+        if (w.rc >= HISTORY_STEP_LIMIT)
+                atomicAdd(a.counters + C_STATUS + (w.rc - HISTORY_STEP_LIMIT),
+                    1ull);
+        else if (w.rc != RETURN_SUCCESS)
                 atomicCAS(a.counters + C_ERROR, 0ull,
                     (unsigned long long)w.rc);
 }
@@ -1294,6 +1300,11 @@ enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
         }
         *total_steps += host[C_STEPS];
         if ((*error == 0) && (host[C_ERROR] != 0)) *error = host[C_ERROR];
+        if (stats) {
+                stats->step_limited += host[C_STATUS];
+                stats->loop_limited += host[C_STATUS_LOOP];
+                stats->no_target += host[C_STATUS_TARGET];
+        }
         void * mem = res.mem;
         res.mem = nullptr;
         HIP_TRY(hipFreeAsync(mem, stream));

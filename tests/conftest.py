@@ -37,6 +37,16 @@ def ref_standard():
 
 
 @pytest.fixture(scope="session")
+def ref_iron():
+    import oracle as orc
+    if not orc.available():
+        pytest.skip("oracle library not built (make -C oracle)")
+    p = orc.RefPhysics("iron_c001")
+    yield p
+    p.close()
+
+
+@pytest.fixture(scope="session")
 def gpu_standard():
     import pumas_amd as pa
     return pa.Physics(pa.physics_path("standard_c005"))
@@ -47,23 +57,3 @@ def gpu_iron():
     import pumas_amd as pa
     return pa.Physics(pa.physics_path("iron_c001"))
 import os, sys; sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
-
-
-@pytest.fixture(scope="session")
-def ref_standard_aligned():
-    import oracle as orc
-    if not orc.available(aligned=True):
-        pytest.skip("aligned oracle library not built (make -C oracle)")
-    p = orc.RefPhysics("standard_c005", aligned=True)
-    yield p
-    p.close()
-
-
-@pytest.fixture(scope="session")
-def ref_iron_aligned():
-    import oracle as orc
-    if not orc.available(aligned=True):
-        pytest.skip("aligned oracle library not built (make -C oracle)")
-    p = orc.RefPhysics("iron_c001", aligned=True)
-    yield p
-    p.close()

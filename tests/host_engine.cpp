@@ -68,6 +68,10 @@ extern "C" int host_load(const char *path) {
 }
 
 static Recorder g_rec;
+static int g_max_steps = 10000000;
+
+/* Step cap of the following host_transport calls. */
+extern "C" void host_set_max_steps(int n) { g_max_steps = n; }
 
 /* Frames of the following host_transport calls go to `frames` (NULL stops
  * recording), as the device recorder. */
@@ -99,7 +103,7 @@ extern "C" int host_transport(const pumas_gpu_context *ctx, const pumas_gpu_geom
         for (int k = 0; k < 3; k++) { t.pos[k] = s->position[k]; t.dir[k] = s->direction[k]; }
         t.decayed = s->decayed; t.step_first = 0; t.step_event = t.step_foreseen = 0;
         t.step_X_limit = t.step_invlb1 = t.lifetime = 0.; t.randn_done = 0; t.randn_next = 0.;
-        t.ix.reset(g_view.n_energies); t.fx.reset(g_view.n_energies); t.rng.init(seed, first + i, STREAM_TRANSPORT, 0); t.steps = 0; t.max_steps = 10000000;
+        t.ix.reset(g_view.n_energies); t.fx.reset(g_view.n_energies); t.rng.init(seed, first + i, STREAM_TRANSPORT, 0); t.steps = 0; t.max_steps = g_max_steps;
         eng.walker_run(w);
         s->charge = t.charge; s->energy = t.energy; s->distance = t.distance; s->grammage = t.grammage;
         s->time = t.time; s->weight = t.weight;

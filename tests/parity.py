@@ -27,7 +27,7 @@ def weight_errors(a, b):
     return np.where(x == y, 0.0, np.abs(x - y) / np.maximum(np.abs(y), 1e-300))
 
 
-def assert_parity(gpu, ref, rtol, rtol_strict=1e-12, frac_strict=0.97,
+def assert_parity(gpu, ref, rtol, rtol_strict=1e-12, frac_strict=1.0,
                   counters=True, weight_rtol=None, max_flips=0.0):
     """Events, media, step and draw counts identical; every history within
     `rtol`, and at least `frac_strict` of them within `rtol_strict`. When
@@ -63,5 +63,10 @@ def assert_parity(gpu, ref, rtol, rtol_strict=1e-12, frac_strict=0.97,
             f"{worst}: flipped={bool(flipped[worst])} errors {detail}")
     keep = ~(flipped | outliers)
     strict = float(np.mean(err[keep] <= rtol_strict)) if keep.any() else 1.0
+    print(f"parity: n={len(final)} branch-flips={int(flipped.sum())} "
+          f"within {rtol_strict:g}: {float(np.mean(err <= rtol_strict)):.6f} "
+          f"max err {float(err.max()) if len(err) else 0.0:.3e} "
+          f"max weight err {float(werr.max()) if len(werr) else 0.0:.3e}",
+          flush=True)
     assert strict >= frac_strict, f"only {strict:.4f} within {rtol_strict}"
     return err

@@ -3,7 +3,7 @@
 Frames of a history are recorded in time order by the lane that carries
 it; frames of different histories interleave. Grouped by history (stable
 order), the device frames must be the reference's: same count, events,
-media and history ids, states at the stepping tolerance.
+media and history ids, states within 1e-12.
 """
 import numpy as np
 import pytest
@@ -12,7 +12,7 @@ import pumas_amd as pa
 from pumas_amd import workloads as W
 
 import make_golden
-from test_gpu_parity import RTOL_STEPPING
+from test_gpu_parity import RTOL
 
 pytestmark = pytest.mark.gpu
 
@@ -25,7 +25,7 @@ def grouped(frames):
 @pytest.mark.parametrize("period", [1, 5, 0])
 @pytest.mark.parametrize("case", ["slabs_straggled", "slabs_backward",
                                   "csda_rock"])
-def test_frames_vs_reference_recorder(gpu_standard, ref_standard_aligned,
+def test_frames_vs_reference_recorder(gpu_standard, ref_standard,
                                       case, period):
     import oracle as orc
     name, pname, ctx, desc, states = next(
@@ -36,8 +36,8 @@ def test_frames_vs_reference_recorder(gpu_standard, ref_standard_aligned,
     gpu = pa.transport(gpu_standard, ctx, pa.Geometry(gpu_standard, desc),
                        states.copy(), recorder=rec)
     frames, count = rec.frames()
-    fin, ref = orc.transport_record(ref_standard_aligned, ctx,
-                                    orc.RefGeometry(desc, aligned=True),
+    fin, ref = orc.transport_record(ref_standard, ctx,
+                                    orc.RefGeometry(desc),
                                     states, period=period,
                                     capacity=1 << 18)
     assert count == ref["count"] == len(frames)
@@ -47,7 +47,7 @@ def test_frames_vs_reference_recorder(gpu_standard, ref_standard_aligned,
     np.testing.assert_array_equal(f["medium"], ref["media"])
     for k in ("energy", "distance", "grammage", "time", "weight"):
         np.testing.assert_allclose(f["state"][k], ref["states"][k],
-                                   rtol=RTOL_STEPPING, atol=1e-300)
+                                   rtol=RTOL, atol=1e-300)
     np.testing.assert_allclose(f["state"]["position"],
                                ref["states"]["position"], rtol=0,
                                atol=1e-6)

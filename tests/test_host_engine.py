@@ -5,10 +5,10 @@ with g++ (no GPU, glibc libm) and runs each history with the same Philox
 stream as the oracle. With the libm of the oracle, every operation of the
 engine must reproduce the reference exactly: final states, events, media,
 step and draw counts are compared for equality on the golden cases and on
-terrain and voxel geometries. The comparison is with the oracle build whose
-float logf/expf are rounded double precision evaluations, the definition the
-engine uses on every platform (dcs_evaluate, src/pumas.c:11243-11347); the
-golden fixtures themselves come from the unmodified reference.
+terrain and voxel geometries. The comparison is with the unmodified reference:
+the engine's math functions (pumas_amd/csrc/libm.h) return the C library's
+bits, including the float logf / expf of dcs_evaluate
+(src/pumas.c:11243-11347).
 """
 import ctypes
 import os
@@ -29,30 +29,40 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(BUILD, "libhost_engine.so")
 
 
-@pytest.fixture(scope="module")
-def host():
+def build_host(name="libhost_engine.so", defines=()):
+    """The engine compiled for the host (g++, glibc; -mfma so that the
+    explicit fma() calls of the math functions are single instructions)."""
     os.makedirs(BUILD, exist_ok=True)
+    lib_path = os.path.join(BUILD, name)
     src = os.path.join(HERE, "host_engine.cpp")
     csrc = os.path.join(REPO, "pumas_amd", "csrc")
     deps = [src] + [os.path.join(csrc, f) for f in os.listdir(csrc)
                     if f.endswith(".h")]
-    if (not os.path.exists(LIB)) or any(
-            os.path.getmtime(d) > os.path.getmtime(LIB) for d in deps):
+    if (not os.path.exists(lib_path)) or any(
+            os.path.getmtime(d) > os.path.getmtime(lib_path) for d in deps):
         subprocess.check_call(
             ["g++", "-O2", "-std=c++17", "-fPIC", "-shared",
-             "-ffp-contract=off", "-I", csrc, "-I",
-             os.path.join(REPO, "include"), "-o", LIB, src])
-    lib = ctypes.CDLL(LIB)
+             "-ffp-contract=off", "-mfma"] + [f"-D{d}" for d in defines] +
+            ["-I", csrc, "-I", os.path.join(REPO, "include"), "-o",
+             lib_path, src])
+    lib = ctypes.CDLL(lib_path)
     lib.host_load.argtypes = [ctypes.c_char_p]
     lib.host_record.argtypes = [ctypes.c_void_p, ctypes.c_void_p,
                                 ctypes.c_ulonglong, ctypes.c_int,
                                 ctypes.c_uint64, ctypes.c_uint64]
     lib.host_record.restype = None
+    lib.host_set_max_steps.argtypes = [ctypes.c_int]
+    lib.host_set_max_steps.restype = None
     lib.host_transport.argtypes = [
         ctypes.POINTER(_abi.GpuContext), ctypes.POINTER(_abi.GeometryDesc),
         ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64,
         ctypes.c_uint64] + [ctypes.c_void_p] * 5
     return lib
+
+
+@pytest.fixture(scope="module")
+def host():
+    return build_host()
 
 
 def run_host(lib, physics, ctx, desc, states, seed=0):
@@ -86,46 +96,56 @@ def check_exact(host_out, ref, skip_aborted=False):
 
 
 @pytest.mark.parametrize("case", [c[0] for c in make_golden.cases()])
-def test_golden_cases_bit_exact(host, ref_standard_aligned, case):
+def test_golden_cases_bit_exact(host, ref_standard, case):
     import oracle as orc
     name, pname, ctx, desc, states = next(
         c for c in make_golden.cases() if c[0] == case)
     states = states[:1024]
     out = run_host(host, pname, ctx, desc, states)
-    ref = orc.transport(ref_standard_aligned, ctx,
-                        orc.RefGeometry(desc, aligned=True), states, seed=0,
+    ref = orc.transport(ref_standard, ctx,
+                        orc.RefGeometry(desc), states, seed=0,
                         threads=8)
     check_exact(out, ref)
 
 
-def test_terrain_bit_exact(host, ref_standard_aligned):
+def test_terrain_bit_exact(host, ref_standard):
     import oracle as orc
     desc = W.terrain_geometry(64, spacing=20.0)
     states = W.terrain_states(300, 63 * 20.0)
     ctx = W.terrain_context()
     out = run_host(host, "standard_c005", ctx, desc, states)
-    ref = orc.transport(ref_standard_aligned, ctx,
-                        orc.RefGeometry(desc, aligned=True), states, seed=0,
+    ref = orc.transport(ref_standard, ctx,
+                        orc.RefGeometry(desc), states, seed=0,
                         threads=8)
     check_exact(out, ref, skip_aborted=True)
 
 
-def test_voxels_bit_exact(host, ref_iron_aligned):
+def test_voxels_bit_exact(host, ref_iron):
+    """Voxels at a 1% cutoff: the reference aborts (assert(0) of its
+    backward target selection, src/pumas.c:6264) on some histories; the
+    engine marks exactly those with PUMAS_GPU_HISTORY_NO_TARGET, and every
+    other history is bit identical."""
     import oracle as orc
     desc = W.voxel_geometry(32, size=8.0)
-    states = W.voxel_states(200, 32 * 8.0)
+    states = W.voxel_states(400, 32 * 8.0)
     ctx = W.voxel_context()
     out = run_host(host, "iron_c001", ctx, desc, states)
-    ref = orc.transport(ref_iron_aligned, ctx,
-                        orc.RefGeometry(desc, aligned=True), states, seed=0,
+    ref = orc.transport(ref_iron, ctx,
+                        orc.RefGeometry(desc), states, seed=0,
                         threads=8)
+    aborted = ref.rc == -1
+    print(f"voxels: {int(aborted.sum())} of {len(states)} reference aborts")
+    assert aborted.any()
+    rc = out[5]
+    np.testing.assert_array_equal(rc == int(pa.HistoryStatus.NO_TARGET),
+                                  aborted)
     check_exact(out, ref, skip_aborted=True)
 
 
 @pytest.mark.parametrize("period", [1, 7, 0])
 @pytest.mark.parametrize("case", ["slabs_straggled", "slabs_backward",
                                   "slabs_mixed", "csda_rock"])
-def test_recorder_frames_bit_exact(host, ref_standard_aligned, case, period):
+def test_recorder_frames_bit_exact(host, ref_standard, case, period):
     """The device recorder's frames (track start and stop, every period-th
     step, DEL pre and post points, medium changes) are the frames of the
     reference's recorder, in order, bit for bit."""
@@ -142,8 +162,8 @@ def test_recorder_frames_bit_exact(host, ref_standard_aligned, case, period):
         out = run_host(host, pname, ctx, desc, states)
     finally:
         host.host_record(None, None, 0, 0, 0, 0)
-    fin, ref = orc.transport_record(ref_standard_aligned, ctx,
-                                    orc.RefGeometry(desc, aligned=True),
+    fin, ref = orc.transport_record(ref_standard, ctx,
+                                    orc.RefGeometry(desc),
                                     states, period=period, capacity=cap)
     assert (out[0] == fin).all()
     k = count.value
@@ -190,3 +210,44 @@ def test_coarse_kinetic_index_matches_bisection(host, physics):
     host.host_check_k_index.argtypes = [ctypes.c_long]
     assert host.host_load(pa.physics_path(physics).encode()) == 0
     assert host.host_check_k_index(1_000_000) == 0
+
+
+def test_step_cap_status(host):
+    """A history that reaches the step cap ends there with no event and
+    rc PUMAS_GPU_HISTORY_STEP_LIMIT (not as a limit event); the others are
+    unchanged."""
+    ctx, desc = W.slabs_forward_context(), W.slabs_geometry()
+    states = W.slabs_forward_states(64, seed=2)
+    ok = run_host(host, "standard_c005", ctx, desc, states)
+    host.host_set_max_steps(200)
+    try:
+        capped = run_host(host, "standard_c005", ctx, desc, states)
+    finally:
+        host.host_set_max_steps(10000000)
+    s, ev, md, st, dr, rc = capped
+    long = ok[3] > 200
+    assert long.any() and (~long).any()
+    assert (rc[long] == int(pa.HistoryStatus.STEP_LIMIT)).all()
+    assert (ev[long] == int(pa.Event.NONE)).all()
+    assert (st[long] == 200).all()
+    assert (rc[~long] == 0).all()
+    assert (s[~long] == ok[0][~long]).all()
+    np.testing.assert_array_equal(ev[~long], ok[1][~long])
+
+
+def test_loop_limit_status():
+    """With the loop limit forced down to 2 iterations (a build of the
+    engine for this test), histories whose rejection loops give up end with
+    rc PUMAS_GPU_HISTORY_LOOP_LIMIT and no event; the histories whose loops
+    never reached the limit are bit identical to the normal build."""
+    normal = build_host()
+    forced = build_host("libhost_engine_loop2.so", ("PG_LOOP_MAX=2",))
+    ctx, desc = W.slabs_forward_context(), W.slabs_geometry()
+    states = W.slabs_forward_states(128, seed=3)
+    ok = run_host(normal, "standard_c005", ctx, desc, states)
+    out = run_host(forced, "standard_c005", ctx, desc, states)
+    flagged = out[5] == int(pa.HistoryStatus.LOOP_LIMIT)
+    assert flagged.any() and (~flagged).any()
+    assert (out[1][flagged] == int(pa.Event.NONE)).all()
+    assert (out[5][~flagged] == 0).all()
+    assert (out[0][~flagged] == ok[0][~flagged]).all()
