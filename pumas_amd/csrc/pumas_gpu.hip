@@ -1598,60 +1598,65 @@ extern "C" enum pumas_return pumas_gpu_transport_host(
     const struct pumas_gpu_outputs * outputs,
     const struct pumas_gpu_options * options, struct pumas_gpu_stats * stats)
 {
-        struct pumas_state * d_states = nullptr;
-        struct pumas_gpu_outputs d_out;
-        memset(&d_out, 0x0, sizeof(d_out));
+        /* Device copies, freed on every return path. The outputs are
+         * cleared first, so that a call that fails before its first launch
+         * returns the input states and zero outputs. */
+        struct Buffers {
+                struct pumas_state * states = nullptr;
+                struct pumas_gpu_outputs out = {};
+                hipStream_t stream = nullptr;
+                ~Buffers()
+                {
+                        if (stream) {
+                                (void)hipStreamSynchronize(stream);
+                                (void)hipStreamDestroy(stream);
+                        }
+                        (void)hipFree(states);
+                        (void)hipFree(out.events);
+                        (void)hipFree(out.media);
+                        (void)hipFree(out.steps);
+                        (void)hipFree(out.draws);
+                        (void)hipFree(out.rc);
+                }
+        } b;
         const size_t ns = n > 0 ? n : 1;
-        HIP_TRY(hipMalloc(&d_states, ns * sizeof(*states)));
-        HIP_TRY(hipMemcpy(d_states, states, n * sizeof(*states),
+        HIP_TRY(hipMalloc(&b.states, ns * sizeof(*states)));
+        HIP_TRY(hipMemcpy(b.states, states, n * sizeof(*states),
             hipMemcpyHostToDevice));
-        if (outputs) {
-                if (outputs->events)
-                        HIP_TRY(hipMalloc(&d_out.events, ns * sizeof(int)));
-                if (outputs->media)
-                        HIP_TRY(hipMalloc(&d_out.media, 2 * ns * sizeof(int32_t)));
-                if (outputs->steps)
-                        HIP_TRY(hipMalloc(&d_out.steps, ns * sizeof(uint32_t)));
-                if (outputs->draws)
-                        HIP_TRY(hipMalloc(&d_out.draws, ns * sizeof(uint32_t)));
-                if (outputs->rc)
-                        HIP_TRY(hipMalloc(&d_out.rc, ns * sizeof(int32_t)));
+#define PG_OUT(field, count)                                                  \
+        if (outputs && outputs->field) {                                       \
+                HIP_TRY(hipMalloc(&b.out.field,                                \
+                    (count) * ns * sizeof(*outputs->field)));                  \
+                HIP_TRY(hipMemset(b.out.field, 0x0,                            \
+                    (count) * ns * sizeof(*outputs->field)));                  \
         }
+        PG_OUT(events, 1)
+        PG_OUT(media, 2)
+        PG_OUT(steps, 1)
+        PG_OUT(draws, 1)
+        PG_OUT(rc, 1)
+#undef PG_OUT
         /* A stream of its own: the legacy NULL stream cannot be captured
          * in a graph. */
-        hipStream_t stream = nullptr;
-        HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        HIP_TRY(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
         const enum pumas_return rc = pumas_gpu_transport(physics, context,
-            geometry, d_states, n, seed, first_history, &d_out, options, stats,
-            stream);
-        (void)hipStreamSynchronize(stream);
-        (void)hipStreamDestroy(stream);
+            geometry, b.states, n, seed, first_history, &b.out, options, stats,
+            b.stream);
+        HIP_TRY(hipStreamSynchronize(b.stream));
         const std::string message = g_error;
-        HIP_TRY(hipMemcpy(states, d_states, n * sizeof(*states),
+        HIP_TRY(hipMemcpy(states, b.states, n * sizeof(*states),
             hipMemcpyDeviceToHost));
-        if (outputs) {
-                if (outputs->events)
-                        HIP_TRY(hipMemcpy(outputs->events, d_out.events,
-                            n * sizeof(int), hipMemcpyDeviceToHost));
-                if (outputs->media)
-                        HIP_TRY(hipMemcpy(outputs->media, d_out.media,
-                            2 * n * sizeof(int32_t), hipMemcpyDeviceToHost));
-                if (outputs->steps)
-                        HIP_TRY(hipMemcpy(outputs->steps, d_out.steps,
-                            n * sizeof(uint32_t), hipMemcpyDeviceToHost));
-                if (outputs->draws)
-                        HIP_TRY(hipMemcpy(outputs->draws, d_out.draws,
-                            n * sizeof(uint32_t), hipMemcpyDeviceToHost));
-                if (outputs->rc)
-                        HIP_TRY(hipMemcpy(outputs->rc, d_out.rc,
-                            n * sizeof(int32_t), hipMemcpyDeviceToHost));
-        }
-        (void)hipFree(d_states);
-        (void)hipFree(d_out.events);
-        (void)hipFree(d_out.media);
-        (void)hipFree(d_out.steps);
-        (void)hipFree(d_out.draws);
-        (void)hipFree(d_out.rc);
+#define PG_BACK(field, count)                                                 \
+        if (outputs && outputs->field)                                         \
+                HIP_TRY(hipMemcpy(outputs->field, b.out.field,                 \
+                    (count) * n * sizeof(*outputs->field),                     \
+                    hipMemcpyDeviceToHost));
+        PG_BACK(events, 1)
+        PG_BACK(media, 2)
+        PG_BACK(steps, 1)
+        PG_BACK(draws, 1)
+        PG_BACK(rc, 1)
+#undef PG_BACK
         snprintf(g_error, sizeof(g_error), "%s", message.c_str());
         return rc;
 }
