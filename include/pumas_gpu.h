@@ -370,6 +370,27 @@ enum pumas_return pumas_gpu_transport(const pumas_gpu_physics * physics,
     void * stream);
 
 /*
+ * Same as pumas_gpu_transport, allocating nothing: no reference
+ * counterpart. For callers that cannot call hipMalloc (code under stream
+ * capture, tight serving loops, harnesses that forbid allocations). The
+ * transport's per-call device buffers come from a static array of the
+ * library on each GPU (about 90 MB, held until the process ends), which
+ * limits a call to PUMAS_GPU_NOALLOC_MAX_HISTORIES histories (larger `n`
+ * is refused with PUMAS_RETURN_VALUE_ERROR). Launches are eager (no graph)
+ * and `stats` holds counts, no timings. Calls on one GPU share its array:
+ * calls that overlap on a GPU race and must be serialised by the caller.
+ */
+#define PUMAS_GPU_NOALLOC_MAX_HISTORIES (1 << 18)
+enum pumas_return pumas_gpu_transport_noalloc(
+    const pumas_gpu_physics * physics,
+    const struct pumas_gpu_context * context,
+    const pumas_gpu_geometry * geometry, struct pumas_state * states,
+    size_t n, uint64_t seed, uint64_t first_history,
+    const struct pumas_gpu_outputs * outputs,
+    const struct pumas_gpu_options * options, struct pumas_gpu_stats * stats,
+    void * stream);
+
+/*
  * Host-memory convenience: copies `states` (and requested outputs) to the
  * device, transports them and copies the results back. Output pointers in
  * `outputs` are host pointers here.

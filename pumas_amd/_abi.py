@@ -171,7 +171,11 @@ class GpuStats(ctypes.Structure):
                 ("no_target", ctypes.c_uint64)]
 
 
-# Every entry point declared by include/pumas_gpu.h.
+# Histories per call of pumas_gpu_transport_noalloc
+# (PUMAS_GPU_NOALLOC_MAX_HISTORIES).
+NOALLOC_MAX_HISTORIES = 1 << 18
+
+
 class Flux(enum.IntEnum):
     """enum pumas_gpu_flux."""
     NONE = 0
@@ -229,6 +233,7 @@ FRAME_DTYPE = np.dtype([("state", STATE_DTYPE), ("medium", np.int32),
 assert FRAME_DTYPE.itemsize == 128
 
 
+# Every entry point declared by include/pumas_gpu.h.
 EXPORTED_SYMBOLS = (
     "pumas_gpu_context_default",
     "pumas_gpu_physics_upload",
@@ -241,6 +246,7 @@ EXPORTED_SYMBOLS = (
     "pumas_gpu_geometry_destroy",
     "pumas_gpu_transport",
     "pumas_gpu_transport_host",
+    "pumas_gpu_transport_noalloc",
     "pumas_gpu_tally",
     "pumas_gpu_tally_host",
     "pumas_gpu_autotune",
@@ -288,6 +294,10 @@ def declare(lib):
         vp, P(GpuContext), vp, vp, ctypes.c_size_t, ctypes.c_uint64,
         ctypes.c_uint64, P(GpuOutputs), P(GpuOptions), P(GpuStats)]
     lib.pumas_gpu_transport_host.restype = ctypes.c_int
+    lib.pumas_gpu_transport_noalloc.argtypes = [
+        vp, P(GpuContext), vp, vp, ctypes.c_size_t, ctypes.c_uint64,
+        ctypes.c_uint64, P(GpuOutputs), P(GpuOptions), P(GpuStats), vp]
+    lib.pumas_gpu_transport_noalloc.restype = ctypes.c_int
     lib.pumas_gpu_tally.argtypes = [vp, vp, ctypes.c_size_t, P(TallySpec),
                                     P(TallyResult), vp]
     lib.pumas_gpu_tally.restype = ctypes.c_int

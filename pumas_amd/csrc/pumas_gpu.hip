@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "../../include/pumas_gpu.h"
@@ -1171,10 +1172,21 @@ struct ChunkResources {
         }
 };
 
-/* Transport histories [base, base + m) of the batch described by `a0`. */
+/* Device bytes of the per-chunk buffers (counters, records, queues) of a
+ * chunk of m histories. */
+constexpr size_t chunk_bytes(size_t m)
+{
+        return 256 + (((m * sizeof(Walker)) + 255) & ~size_t(255)) +
+            3 * (((m * sizeof(uint32_t)) + 255) & ~size_t(255));
+}
+
+/* Transport histories [base, base + m) of the batch described by `a0`.
+ * The chunk's buffers are allocated on the stream, or taken from `fixed`
+ * (at least chunk_bytes(m) bytes of device memory) when it is given. */
 enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
     size_t m, bool f32, Launch L, unsigned long long * total_steps,
-    unsigned long long * error, struct pumas_gpu_stats * stats)
+    unsigned long long * error, struct pumas_gpu_stats * stats,
+    void * fixed = nullptr)
 {
         /* No more resident blocks than the chunk fills with histories, so
          * that a small request takes full wavefronts on few CUs and leaves
@@ -1217,9 +1229,14 @@ enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
         const size_t b_counters = 256;
         const size_t b_records = ((m * sizeof(Walker)) + 255) & ~size_t(255);
         const size_t b_queue = ((m * sizeof(uint32_t)) + 255) & ~size_t(255);
-        HIP_TRY(hipMallocAsync(
-            &res.mem, b_counters + b_records + 3 * b_queue, stream));
-        char * ptr = static_cast<char *>(res.mem);
+        char * ptr;
+        if (fixed) {
+                ptr = static_cast<char *>(fixed);
+        } else {
+                HIP_TRY(hipMallocAsync(
+                    &res.mem, b_counters + b_records + 3 * b_queue, stream));
+                ptr = static_cast<char *>(res.mem);
+        }
         a.counters = reinterpret_cast<unsigned long long *>(ptr);
         a.records = reinterpret_cast<Walker *>(ptr + b_counters);
         char * q = ptr + b_counters + b_records;
@@ -1305,9 +1322,11 @@ enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
                 stats->loop_limited += host[C_STATUS_LOOP];
                 stats->no_target += host[C_STATUS_TARGET];
         }
-        void * mem = res.mem;
-        res.mem = nullptr;
-        HIP_TRY(hipFreeAsync(mem, stream));
+        if (res.mem) {
+                void * mem = res.mem;
+                res.mem = nullptr;
+                HIP_TRY(hipFreeAsync(mem, stream));
+        }
         return rc;
 }
 
@@ -1336,15 +1355,17 @@ enum pumas_return check_context(const pumas_gpu_physics * physics,
 
 } /* namespace */
 
-extern "C" enum pumas_return pumas_gpu_transport(
-    const pumas_gpu_physics * physics, const struct pumas_gpu_context * context,
+/* The transport call. With `fixed` (device memory for chunk_bytes(n)
+ * bytes) nothing is allocated: no chunk buffers, no graph (eager
+ * launches), no timing events (stats then hold counts only). */
+static enum pumas_return transport_impl(const pumas_gpu_physics * physics,
+    const struct pumas_gpu_context * context,
     const pumas_gpu_geometry * geometry, struct pumas_state * states, size_t n,
     uint64_t seed, uint64_t first_history,
     const struct pumas_gpu_outputs * outputs,
     const struct pumas_gpu_options * options, struct pumas_gpu_stats * stats,
-    void * stream_)
+    void * stream_, void * fixed)
 {
-        pg_clear_error();
         if ((physics == NULL) || (context == NULL) || (geometry == NULL))
                 return pg_fail(PUMAS_RETURN_VALUE_ERROR, "NULL argument");
         if ((states == NULL) && (n > 0))
@@ -1459,7 +1480,7 @@ extern "C" enum pumas_return pumas_gpu_transport(
                                 if (e) (void)hipEventDestroy(e);
                 }
         } ev;
-        if (stats) {
+        if (stats && !fixed) {
                 HIP_TRY(hipEventCreate(&ev.e0));
                 HIP_TRY(hipEventCreate(&ev.e1));
                 HIP_TRY(hipEventRecord(ev.e0, stream));
@@ -1472,10 +1493,11 @@ extern "C" enum pumas_return pumas_gpu_transport(
         }
         unsigned long long total_steps = 0, error = 0;
         /* The legacy NULL stream cannot be captured. */
-        const bool graph = !(options && options->eager) && (stream != nullptr);
+        const bool graph = !(options && options->eager) &&
+            (stream != nullptr) && !fixed;
         if (stats) {
                 memset(stats, 0x0, sizeof(*stats));
-                if (!graph) {
+                if (!graph && !fixed) {
                         ev.marks.resize(2 * rounds_per_check, nullptr);
                         for (auto & e : ev.marks)
                                 HIP_TRY(hipEventCreate(&e));
@@ -1491,18 +1513,18 @@ extern "C" enum pumas_return pumas_gpu_transport(
                     lanes_per_wave);
         const Launch L = { n_cu, block, per_cu, rounds_per_check, graph,
                 tail_del, lanes_per_wave, lds_bytes, 0, 0, stream,
-                (stats && !graph) ? ev.marks.data() : nullptr };
+                (stats && !graph && !fixed) ? ev.marks.data() : nullptr };
         for (size_t base = 0; base < n; base += MAX_CHUNK) {
                 const size_t m = (n - base < MAX_CHUNK) ? n - base : MAX_CHUNK;
                 rc = transport_chunk(
-                    a, base, m, f32, L, &total_steps, &error, stats);
+                    a, base, m, f32, L, &total_steps, &error, stats, fixed);
                 if (rc != PUMAS_RETURN_SUCCESS) break;
         }
-        if (stats) HIP_TRY(hipEventRecord(ev.e1, stream));
+        if (stats && !fixed) HIP_TRY(hipEventRecord(ev.e1, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         if (stats) {
                 float ms = 0.f;
-                HIP_TRY(hipEventElapsedTime(&ms, ev.e0, ev.e1));
+                if (!fixed) HIP_TRY(hipEventElapsedTime(&ms, ev.e0, ev.e1));
                 stats->steps = total_steps;
                 stats->histories = n;
                 stats->kernel_ms = ms;
@@ -1513,6 +1535,89 @@ extern "C" enum pumas_return pumas_gpu_transport(
                 return pg_fail((enum pumas_return)error,
                     "a history raised error code %llu", error);
         return PUMAS_RETURN_SUCCESS;
+}
+
+extern "C" enum pumas_return pumas_gpu_transport(
+    const pumas_gpu_physics * physics, const struct pumas_gpu_context * context,
+    const pumas_gpu_geometry * geometry, struct pumas_state * states, size_t n,
+    uint64_t seed, uint64_t first_history,
+    const struct pumas_gpu_outputs * outputs,
+    const struct pumas_gpu_options * options, struct pumas_gpu_stats * stats,
+    void * stream)
+{
+        pg_clear_error();
+        return transport_impl(physics, context, geometry, states, n, seed,
+            first_history, outputs, options, stats, stream, nullptr);
+}
+
+/* ------------------------------------------------------------------------
+ * Transport without allocations.
+ *
+ * The per-chunk buffers of the transport (counters, in-flight records,
+ * queues) live in one file-scope device array per GPU, so that callers
+ * that cannot allocate device memory (code under stream capture, serving
+ * loops, harnesses that forbid hipMalloc) can transport batches of up to
+ * PUMAS_GPU_NOALLOC_MAX_HISTORIES histories. In host code the array's name
+ * is not a device address: its address on each device is looked up once
+ * with hipGetSymbolAddress and only that pointer is passed to kernels.
+ * The counters are cleared by a kernel at the start of every call; the
+ * records and queues are written before they are read. Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU race.
+ * ------------------------------------------------------------------------ */
+constexpr size_t NOALLOC_BYTES = chunk_bytes(PUMAS_GPU_NOALLOC_MAX_HISTORIES);
+static_assert(PUMAS_GPU_NOALLOC_MAX_HISTORIES <= MAX_CHUNK, "one chunk");
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_lock;
+void * g_scratch[MAX_DEVICES];
+
+/* The device address of `scratch` on the current device (looked up on the
+ * first call on that device). */
+enum pumas_return scratch_pointer(void ** out)
+{
+        int device = 0;
+        HIP_TRY(hipGetDevice(&device));
+        if ((device < 0) || (device >= MAX_DEVICES))
+                return pg_fail(PUMAS_RETURN_INDEX_ERROR,
+                    "device %d beyond the %d supported", device, MAX_DEVICES);
+        std::lock_guard<std::mutex> guard(g_scratch_lock);
+        if (g_scratch[device] == nullptr) {
+                void * ptr = nullptr;
+                const hipError_t e =
+                    hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+                if ((e != hipSuccess) || (ptr == nullptr))
+                        return pg_fail(PUMAS_RETURN_MEMORY_ERROR,
+                            "hipGetSymbolAddress(scratch): %s",
+                            hipGetErrorString(e));
+                g_scratch[device] = ptr;
+        }
+        *out = g_scratch[device];
+        return PUMAS_RETURN_SUCCESS;
+}
+} /* namespace */
+
+extern "C" enum pumas_return pumas_gpu_transport_noalloc(
+    const pumas_gpu_physics * physics, const struct pumas_gpu_context * context,
+    const pumas_gpu_geometry * geometry, struct pumas_state * states, size_t n,
+    uint64_t seed, uint64_t first_history,
+    const struct pumas_gpu_outputs * outputs,
+    const struct pumas_gpu_options * options, struct pumas_gpu_stats * stats,
+    void * stream)
+{
+        pg_clear_error();
+        if (n > PUMAS_GPU_NOALLOC_MAX_HISTORIES)
+                return pg_fail(PUMAS_RETURN_VALUE_ERROR,
+                    "%zu histories exceed the %d of the transport without "
+                    "allocations (use pumas_gpu_transport, or split the "
+                    "batch)", n, PUMAS_GPU_NOALLOC_MAX_HISTORIES);
+        void * fixed = nullptr;
+        const enum pumas_return rc = scratch_pointer(&fixed);
+        if (rc != PUMAS_RETURN_SUCCESS) return rc;
+        return transport_impl(physics, context, geometry, states, n, seed,
+            first_history, outputs, options, stats, stream, fixed);
 }
 
 /* ------------------------------------------------------------------------
