@@ -12,7 +12,11 @@ before the timed region.
 
 The reported value is muon-steps per second (one step = one Monte Carlo
 step of pumas_context_transport, i.e. one geometry step proposal); events
-per second are reported beside it. The CPU baseline is the reference library
+per second are reported beside it. The timed passes launch the scheduler's
+rounds one by one, with HIP events around each stepping kernel launch (the
+roofline's kernel time); after the timed region the same batch is run a
+few passes more as HIP graph replays (the library's default launch path,
+`graph_replay`) and in FP32-bulk (`fp32_bulk`). The CPU baseline is the reference library
 (compiled from its source by oracle/Makefile) with one context per thread on
 the host cores, over a bounded sample of the same workload.
 
@@ -92,6 +96,11 @@ def parse():
                     help="arrival window of the serving run (Poisson "
                          "requests at 200/s); 0 skips it")
     ap.add_argument("--serve-slots", type=int, default=0)
+    ap.add_argument("--extra-passes", type=int, default=3,
+                    help="passes of each side measurement after the timed "
+                         "region of the slab workload: the same batch "
+                         "replayed as HIP graphs (the library's default "
+                         "launch path) and in FP32-bulk (0 skips them)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this "
                          "many MiB (calibrates HBM byte counters)")
@@ -112,13 +121,39 @@ def copy_calibration(mib):
                       "launches": 3}))
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_threads():
+    """Threads of the CPU baseline: every core of the affinity mask, capped
+    by OMP_NUM_THREADS when it is set (a shared GPU box gives each GPU a
+    share of its host cores and states it there)."""
+    affinity = len(os.sched_getaffinity(0))
+    share = os.environ.get("OMP_NUM_THREADS")
+    threads = affinity
+    note = f"all {affinity} cores of the affinity mask"
+    if share and share.isdigit() and 0 < int(share) < affinity:
+        threads = int(share)
+        note = (f"this GPU's share of the host ({threads} cores, "
+                f"OMP_NUM_THREADS) of the {affinity} in the affinity mask")
+    return threads, note
+
+
 def cpu_baseline(states, seconds, physics, desc, ctx):
     """Reference CPU path, one context per thread, on a bounded sample."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import oracle as orc
     if not orc.available():
         return None
-    threads = min(16, len(os.sched_getaffinity(0)))
+    threads, cores_note = cpu_threads()
     phys = orc.RefPhysics(physics)
     geom = orc.RefGeometry(desc)
     # Calibrate on a small sample, then size the sample to `seconds`.
@@ -133,6 +168,8 @@ def cpu_baseline(states, seconds, physics, desc, ctx):
         "unit": "muon-steps/s",
         "events_per_s": n / r.seconds,
         "cores": threads,
+        "cores_note": cores_note,
+        "cpu_model": cpu_model(),
         "kind": "reference",
         "sample": f"{n} histories of the same workload ({r.total_steps} "
                   f"steps, {r.seconds:.1f} s)",
@@ -361,16 +398,18 @@ def main():
     staged = []
     restore_in_loop = pristine.numel() * args.steps > (32 << 30)
 
-    def one_pass(k, buf=None):
+    def one_pass(k, buf=None, prec=None, eager=None):
         if buf is None:
             work.copy_(pristine)
             buf = work
         st = pa.transport_device(
             phys, ctx, geom, buf.data_ptr(), n, seed=0,
-            first_history=k * n, stream=stream, precision=precision,
+            first_history=k * n, stream=stream,
+            precision=precision if prec is None else prec,
             block_size=tuned["block_size"],
             blocks_per_cu=tuned["blocks_per_cu"],
-            ehs_batch=tuned["ehs_batch"], eager=not args.graph,
+            ehs_batch=tuned["ehs_batch"],
+            eager=(not args.graph) if eager is None else eager,
             lds_tables=args.lds_tables, max_steps=args.max_steps,
             tail_del=args.tail_del)
         if flux_spec is not None:
@@ -382,9 +421,9 @@ def main():
         st = one_pass(1000 + k)
         log(f"bench: warmup pass {k}: {st.kernel_ms:.1f} ms, {st.steps} steps")
 
+    staged = []
     if not restore_in_loop:
         staged = [pristine.clone() for _ in range(args.steps)]
-        del work
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     passes = []
@@ -396,6 +435,34 @@ def main():
             f"{st.step_launches} launches")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    del staged
+
+    def side(label, prec, eager):
+        """Passes of the same batch after the timed region, timed the same
+        way (wall clock between two device synchronisations)."""
+        if args.extra_passes <= 0 or args.workload != "slabs":
+            return None
+        k_passes = min(args.extra_passes, args.steps)
+        bufs = [pristine.clone() for _ in range(k_passes)] \
+            if not restore_in_loop else [None] * k_passes
+        one_pass(2000, prec=prec, eager=eager)  # warm-up of this mode
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        res = [one_pass(3000 + j, bufs[j], prec=prec, eager=eager)
+               for j in range(k_passes)]
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t1
+        del bufs
+        steps = sum(st.steps for st in res)
+        log(f"bench: {label}: {k_passes} passes, {steps / dt:.4e} steps/s")
+        return {"value": steps / dt, "unit": "muon-steps/s",
+                "events_per_s": k_passes * n / dt,
+                "ms_per_step": 1e3 * dt / k_passes, "passes": k_passes}
+
+    graph_replay = side("graph replay", precision, False) \
+        if not args.graph else None
+    fp32_bulk = side("FP32-bulk", pa.Precision.FP32_BULK, not args.graph) \
+        if args.precision == "fp64" else None
 
     events_per_s = args.steps * n / elapsed
     mean_ms = sum(st.kernel_ms for st in passes) / len(passes)
@@ -418,14 +485,28 @@ def main():
             kernel_ms = sum(st.kernel_ms for st in passes)
         unit_bytes = BYTES_PER_STEP_FP64
     steps_per_s = total_steps / elapsed
+    if fp32_bulk:
+        fp32_bulk["vs_fp64"] = fp32_bulk["value"] / steps_per_s
+    if graph_replay:
+        graph_replay["vs_eager"] = graph_replay["value"] / steps_per_s
     achieved = total_steps * unit_bytes / (kernel_ms * 1e-3) / 1e9
 
+    # HBM bytes per launch of the stepping kernel, from the committed
+    # FETCH_SIZE / WRITE_SIZE profile of this workload, used only when the
+    # profile was taken with the launch settings timed here.
     traffic = None
+    traffic_note = None
     tpath = os.path.join(HERE, "profiles", "k_step_slabs_traffic.json")
     if (args.workload == "slabs" and os.path.exists(tpath) and
             n == 10_000_000 and args.precision == "fp64"):
         with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            prof = json.load(f)
+        if prof.get("launch") == tuned:
+            traffic = prof.get("bytes_per_launch")
+            traffic_note = prof.get("note")
+        else:
+            traffic_note = (f"profile launch {prof.get('launch')} differs "
+                            f"from the timed one {tuned}")
 
     serve = None
     if args.serve_seconds > 0 and args.workload == "slabs":
@@ -482,8 +563,19 @@ def main():
             "kernel_avg_ms": kernel_ms / max(launches, 1),
             "launches_per_pass": launches / args.steps,
             "bytes_per_unit": unit_bytes,
+            "traffic_note": traffic_note,
+            "measured_bytes_per_unit": (
+                traffic * launches / total_steps if traffic else None),
         },
         "transport_ms_per_pass": mean_ms,
+        "history_status": {
+            "step_limited": sum(st.step_limited for st in passes),
+            "loop_limited": sum(st.loop_limited for st in passes),
+            "no_target": sum(st.no_target for st in passes),
+            "note": "histories of the timed passes ended with a status "
+                    "code (enum pumas_gpu_history_status)"},
+        "graph_replay": graph_replay,
+        "fp32_bulk": fp32_bulk,
         "cpu_baseline": cpu,
         "serving": serve,
     }

@@ -151,6 +151,20 @@ struct pumas_gpu_context {
 /* Defaults of pumas_context_create (src/pumas.c:2582-2605) for a muon. */
 void pumas_gpu_context_default(struct pumas_gpu_context * context);
 
+/* The reference's simulation context (include/pumas.h:636-672); only its
+ * value part is read here. */
+struct pumas_context;
+
+/*
+ * Copies the mode, event, limit and accuracy of a reference context
+ * (created by pumas_context_create, include/pumas.h:1214, or any struct of
+ * its public layout) into `out`. Its callbacks, recorder and user data are
+ * host pointers without a device counterpart: media come from the device
+ * geometry and random numbers from the Philox stream.
+ */
+void pumas_gpu_context_from_reference(const struct pumas_context * context,
+    struct pumas_gpu_context * out);
+
 /* ------------------------------------------------------------------------
  * Physics tables.
  * ------------------------------------------------------------------------ */
@@ -384,6 +398,20 @@ enum pumas_return pumas_gpu_transport(const pumas_gpu_physics * physics,
 enum pumas_return pumas_gpu_transport_noalloc(
     const pumas_gpu_physics * physics,
     const struct pumas_gpu_context * context,
+    const pumas_gpu_geometry * geometry, struct pumas_state * states,
+    size_t n, uint64_t seed, uint64_t first_history,
+    const struct pumas_gpu_outputs * outputs,
+    const struct pumas_gpu_options * options, struct pumas_gpu_stats * stats,
+    void * stream);
+
+/*
+ * pumas_gpu_transport with the reference's own context type, the boundary
+ * of include/pumas.h:1046-1048 for a batch: reads the mode, event, limit and
+ * accuracy of `context` (pumas_gpu_context_from_reference) and transports
+ * the device-resident `states` as pumas_gpu_transport does.
+ */
+enum pumas_return pumas_gpu_transport_context(
+    const pumas_gpu_physics * physics, const struct pumas_context * context,
     const pumas_gpu_geometry * geometry, struct pumas_state * states,
     size_t n, uint64_t seed, uint64_t first_history,
     const struct pumas_gpu_outputs * outputs,

@@ -247,6 +247,8 @@ EXPORTED_SYMBOLS = (
     "pumas_gpu_transport",
     "pumas_gpu_transport_host",
     "pumas_gpu_transport_noalloc",
+    "pumas_gpu_transport_context",
+    "pumas_gpu_context_from_reference",
     "pumas_gpu_tally",
     "pumas_gpu_tally_host",
     "pumas_gpu_autotune",
@@ -298,6 +300,12 @@ def declare(lib):
         vp, P(GpuContext), vp, vp, ctypes.c_size_t, ctypes.c_uint64,
         ctypes.c_uint64, P(GpuOutputs), P(GpuOptions), P(GpuStats), vp]
     lib.pumas_gpu_transport_noalloc.restype = ctypes.c_int
+    lib.pumas_gpu_transport_context.argtypes = [
+        vp, vp, vp, vp, ctypes.c_size_t, ctypes.c_uint64,
+        ctypes.c_uint64, P(GpuOutputs), P(GpuOptions), P(GpuStats), vp]
+    lib.pumas_gpu_transport_context.restype = ctypes.c_int
+    lib.pumas_gpu_context_from_reference.argtypes = [vp, P(GpuContext)]
+    lib.pumas_gpu_context_from_reference.restype = None
     lib.pumas_gpu_tally.argtypes = [vp, vp, ctypes.c_size_t, P(TallySpec),
                                     P(TallyResult), vp]
     lib.pumas_gpu_tally.restype = ctypes.c_int

@@ -110,6 +110,38 @@ PG_FN bool loop_stalled(Track & t, int & iterations)
         return true;
 }
 
+/* Appends a frame of `t` to the recorder. Out of line, with its arguments
+ * copied: neither the history nor the engine object is pinned to memory on
+ * the paths that do not record. */
+static PG_NOINLINE void record_state(const Recorder rec, const Track t,
+    int medium,
+    int event)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+        const unsigned long long k = atomicAdd(rec.count, 1ull);
+#else
+        const unsigned long long k = (*rec.count)++;
+#endif
+        if (k >= rec.capacity) return;
+        Frame & f = rec.frames[k];
+        f.charge = t.charge;
+        f.energy = t.energy;
+        f.distance = t.distance;
+        f.grammage = t.grammage;
+        f.time = t.time;
+        f.weight = t.weight;
+        for (int j = 0; j < 3; j++) {
+                f.position[j] = t.pos[j];
+                f.direction[j] = t.dir[j];
+        }
+        f.decayed = t.decayed;
+        f.medium = medium;
+        f.event = event;
+        f.pad = 0;
+        f.history = ((uint64_t)t.rng.h1 << 32) | t.rng.h0;
+        f.reserved = 0;
+}
+
 /*
  * The transport engine. With F32 false every operation is the reference's
  * FP64 arithmetic. With F32 true (FP32-bulk) the per-step table lookups,
@@ -132,41 +164,10 @@ struct EngineT {
 
         /* Records the state of `w` in `medium` with `event`, if the history
          * is recorded (record_state, src/pumas.c:7738-7780). */
-        PG_NOINLINE void record(const Walker & w, const Track & t, int medium,
-            int event) const
-        {
-#ifdef __HIP_DEVICE_COMPILE__
-                const unsigned long long k = atomicAdd(rec.count, 1ull);
-#else
-                const unsigned long long k = (*rec.count)++;
-#endif
-                if (k >= rec.capacity) return;
-                Frame & f = rec.frames[k];
-                f.charge = t.charge;
-                f.energy = t.energy;
-                f.distance = t.distance;
-                f.grammage = t.grammage;
-                f.time = t.time;
-                f.weight = t.weight;
-                for (int j = 0; j < 3; j++) {
-                        f.position[j] = t.pos[j];
-                        f.direction[j] = t.dir[j];
-                }
-                f.decayed = t.decayed;
-                f.medium = medium;
-                f.event = event;
-                f.pad = 0;
-                f.history = ((uint64_t)t.rng.h1 << 32) | t.rng.h0;
-                f.reserved = 0;
-                (void)w;
-        }
-
-        /* The same, on a copy: keeps the Walker out of memory. */
         PG_FN void record_frame(const Walker & w, int medium, int event) const
         {
                 if (!w.record) return;
-                const Walker copy = w;
-                record(copy, copy.t, medium, event);
+                record_state(rec, w.t, medium, event);
         }
 
         /* Per-step table lookups, in the bulk precision. */

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -760,11 +761,10 @@ struct KernelArgs {
         const KernelArgs & a =                                                 \
             *(const KernelArgs *)__builtin_amdgcn_kernarg_segment_ptr()
 
-/* Load history i into a walker. */
+/* Load history i, whose state is at `s`, into a walker. */
 __device__ __forceinline__ void walker_load(const KernelArgs & a, uint64_t i,
-    Walker & w)
+    Walker & w, const struct pumas_state * s)
 {
-        const struct pumas_state * s = a.states + i;
         Track & t = w.t;
         t.charge = s->charge;
         t.energy = s->energy;
@@ -791,11 +791,10 @@ __device__ __forceinline__ void walker_load(const KernelArgs & a, uint64_t i,
         t.max_steps = a.max_steps;
 }
 
-/* Store the end state and outputs of history i. */
+/* Store the end state of history i at `s` and its outputs. */
 __device__ __forceinline__ void walker_store(const KernelArgs & a, uint64_t i,
-    const Walker & w)
+    const Walker & w, struct pumas_state * s)
 {
-        struct pumas_state * s = a.states + i;
         const Track & t = w.t;
         s->charge = t.charge;
         s->energy = t.energy;
@@ -855,22 +854,43 @@ __device__ __forceinline__ void steps_flush(
         if ((threadIdx.x & 63) == 0 && acc) atomicAdd(a.counters + C_STEPS, acc);
 }
 
-/* Entry of every history of the batch; one lane per history. */
+/* Entry of every history of the batch; one lane per history. The
+ * block's states (104 B records, 26 KB per block) are staged through LDS:
+ * read from HBM with consecutive lanes on consecutive 8 B words, and
+ * written back the same way (unchanged for the histories that go on to
+ * the stepping kernel), instead of each lane touching its own record with
+ * 104 B strides. */
+constexpr int INIT_BLOCK = 256;
+
 template <bool F32>
-__global__ void __launch_bounds__(256) k_init(KernelArgs args)
+__global__ void __launch_bounds__(INIT_BLOCK) k_init(KernelArgs args)
 {
         KERNEL_ARGS(args);
+        __shared__ struct pumas_state stage[INIT_BLOCK];
         const EngineT<F32> eng = { a.p, a.mv, a.c, a.g, a.ft, a.p, a.rec };
-        const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const uint64_t base = (uint64_t)blockIdx.x * INIT_BLOCK;
+        const uint64_t i = base + threadIdx.x;
         const bool active = i < a.n;
+        const uint32_t words = (uint32_t)(
+            ((a.n - base < INIT_BLOCK) ? a.n - base : INIT_BLOCK) *
+            (sizeof(struct pumas_state) / sizeof(double)));
+        double * const lds = reinterpret_cast<double *>(stage);
+        double * const hbm = reinterpret_cast<double *>(a.states + base);
+        for (uint32_t k = threadIdx.x; k < words; k += INIT_BLOCK)
+                lds[k] = hbm[k];
+        __syncthreads();
         Walker w;
         w.phase = PHASE_EMPTY;
         if (active) {
-                walker_load(a, i, w);
+                walker_load(a, i, w, stage + threadIdx.x);
                 eng.walker_init(w);
-                if (w.phase == PHASE_DONE) walker_store(a, i, w);
+                if (w.phase == PHASE_DONE)
+                        walker_store(a, i, w, stage + threadIdx.x);
                 if (w.phase == PHASE_STEP) a.records[i] = w;
         }
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < words; k += INIT_BLOCK)
+                hbm[k] = lds[k];
         queue_append(a.counters + C_STEP_N0, a.q_step[0],
             w.phase == PHASE_STEP, (uint32_t)i);
 }
@@ -960,7 +980,7 @@ __global__ void __launch_bounds__(512, 2) k_step(KernelArgs args, int parity)
                 const bool to_del =
                     (w.phase == PHASE_DEL) && !(TAIL_DEL && exhausted);
                 if (done) {
-                        walker_store(a, slot, w);
+                        walker_store(a, slot, w, a.states + slot);
                         steps_acc += w.t.steps;
                 }
                 if (to_del) a.records[slot] = w;
@@ -1045,7 +1065,7 @@ __global__ void __launch_bounds__(256, 2) k_del(KernelArgs args, int parity)
                         w = a.records[slot];
                         eng.walker_del(w);
                         if (w.phase == PHASE_DONE) {
-                                walker_store(a, slot, w);
+                                walker_store(a, slot, w, a.states + slot);
                                 steps_acc += w.t.steps;
                         } else {
                                 a.records[slot] = w;
@@ -1248,13 +1268,13 @@ enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
         unsigned long long host[C_COUNT];
         hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, stream,
             a.counters, C_COUNT);
-        const unsigned blocks = (unsigned)((m + 255) / 256);
+        const unsigned blocks = (unsigned)((m + INIT_BLOCK - 1) / INIT_BLOCK);
         if (f32)
-                hipLaunchKernelGGL(k_init<true>, dim3(blocks), dim3(256), 0,
-                    stream, a);
+                hipLaunchKernelGGL(k_init<true>, dim3(blocks),
+                    dim3(INIT_BLOCK), 0, stream, a);
         else
-                hipLaunchKernelGGL(k_init<false>, dim3(blocks), dim3(256), 0,
-                    stream, a);
+                hipLaunchKernelGGL(k_init<false>, dim3(blocks),
+                    dim3(INIT_BLOCK), 0, stream, a);
         if (hipGetLastError() != hipSuccess)
                 return pg_fail(PUMAS_RETURN_VALUE_ERROR,
                     "transport kernel launch failed");
@@ -1547,6 +1567,51 @@ extern "C" enum pumas_return pumas_gpu_transport(
 {
         pg_clear_error();
         return transport_impl(physics, context, geometry, states, n, seed,
+            first_history, outputs, options, stats, stream, nullptr);
+}
+
+/* Public layout of the reference's struct pumas_context
+ * (include/pumas.h:636-672): four host pointers (medium and random
+ * callbacks, recorder, user data), then the value part, laid out as
+ * struct pumas_gpu_context (mode, event, limit, accuracy). */
+struct reference_context_layout {
+        void * medium;
+        void * random;
+        void * recorder;
+        void * user_data;
+        struct pumas_gpu_context values;
+};
+static_assert(offsetof(reference_context_layout, values) == 32,
+    "reference context: four pointers before its mode");
+static_assert(offsetof(struct pumas_gpu_context, event) == 16 &&
+        offsetof(struct pumas_gpu_context, limit) == 24 &&
+        offsetof(struct pumas_gpu_context, accuracy) == 56,
+    "value part of the reference context");
+
+extern "C" void pumas_gpu_context_from_reference(
+    const struct pumas_context * context, struct pumas_gpu_context * out)
+{
+        if ((context == NULL) || (out == NULL)) return;
+        memcpy(out,
+            &reinterpret_cast<const reference_context_layout *>(context)
+                 ->values,
+            sizeof(*out));
+}
+
+extern "C" enum pumas_return pumas_gpu_transport_context(
+    const pumas_gpu_physics * physics, const struct pumas_context * context,
+    const pumas_gpu_geometry * geometry, struct pumas_state * states, size_t n,
+    uint64_t seed, uint64_t first_history,
+    const struct pumas_gpu_outputs * outputs,
+    const struct pumas_gpu_options * options, struct pumas_gpu_stats * stats,
+    void * stream)
+{
+        pg_clear_error();
+        if (context == NULL)
+                return pg_fail(PUMAS_RETURN_VALUE_ERROR, "NULL context");
+        struct pumas_gpu_context values;
+        pumas_gpu_context_from_reference(context, &values);
+        return transport_impl(physics, &values, geometry, states, n, seed,
             first_history, outputs, options, stats, stream, nullptr);
 }
 

@@ -118,3 +118,49 @@ def test_ctypes_mirrors_match_the_header(tmp_path):
             assert ctypes.sizeof(cls) == int(value), line
         else:
             assert getattr(cls, what).offset == int(value), line
+
+
+class ReferenceContext(ctypes.Structure):
+    """Public layout of the reference's struct pumas_context
+    (include/pumas.h:636-672): four host pointers, then the value part."""
+    _fields_ = [("medium", ctypes.c_void_p), ("random", ctypes.c_void_p),
+                ("recorder", ctypes.c_void_p), ("user_data", ctypes.c_void_p),
+                ("mode", _abi.ContextMode), ("event", ctypes.c_int),
+                ("limit", _abi.ContextLimit), ("accuracy", ctypes.c_double)]
+
+
+def test_context_from_reference():
+    """The value part of a reference context is read at the reference's
+    offsets (no device work)."""
+    lib = pa.library()
+    r = ReferenceContext()
+    r.medium, r.random, r.recorder, r.user_data = 1, 2, 3, 4
+    r.mode.energy_loss, r.mode.decay = pa.Mode.STRAGGLED, pa.Mode.RANDOMISED
+    r.mode.direction, r.mode.scattering = pa.Mode.BACKWARD, pa.Mode.MIXED
+    r.event = int(pa.Event.LIMIT_DISTANCE | pa.Event.MEDIUM)
+    r.limit.energy, r.limit.distance = 0.5, 1234.5
+    r.limit.grammage, r.limit.time = 6.5e5, 3e-6
+    r.accuracy = 0.02
+    c = _abi.GpuContext()
+    lib.pumas_gpu_context_from_reference(ctypes.byref(r), ctypes.byref(c))
+    assert (c.mode.energy_loss, c.mode.decay, c.mode.direction,
+            c.mode.scattering) == (2, 1, 1, 1)
+    assert c.event == r.event
+    assert (c.limit.energy, c.limit.distance, c.limit.grammage,
+            c.limit.time) == (0.5, 1234.5, 6.5e5, 3e-6)
+    assert c.accuracy == 0.02
+
+
+def test_c_caller_compiles_against_reference_header():
+    """tests/abi_check.c includes the reference's include/pumas.h before
+    pumas_gpu.h and static-asserts the shared layouts and enum values; it
+    is built (tests/kernels/Makefile) wherever the reference is present."""
+    ref = "/root/reference/include/pumas.h"
+    if not os.path.exists(ref):
+        pytest.skip("reference header not present")
+    exe = os.path.join(REPO, "tests", "_build", "abi_check")
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C",
+                           os.path.join(REPO, "tests", "kernels"),
+                           "../_build/abi_check"])
+    assert os.access(exe, os.X_OK)
