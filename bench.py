@@ -468,11 +468,12 @@ def main():
     mean_ms = sum(st.kernel_ms for st in passes) / len(passes)
     if csda:
         # One muon-step per history on the closed form path; the dominant
-        # kernel is k_init (one launch per chunk), timed with the call.
+        # kernel is k_init (one launch per chunk of 2^25 histories), timed
+        # with HIP events around each launch on its stream.
         total_steps = args.steps * n
         kernel, launches = "k_init", args.steps * ((n + (1 << 25) - 1) >>
                                                    25)
-        kernel_ms = sum(st.kernel_ms for st in passes)
+        kernel_ms = sum(st.init_kernel_ms for st in passes)
         unit_bytes = BYTES_PER_HISTORY_CSDA
     else:
         # Roofline of the stepping kernel: algorithmic bytes of its

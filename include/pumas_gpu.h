@@ -325,8 +325,9 @@ enum pumas_gpu_history_status {
         /* options->max_steps Monte Carlo steps were taken (the reference has
          * no cap). */
         PUMAS_GPU_HISTORY_STEP_LIMIT = 100,
-        /* A rejection or bisection loop reached 65536 iterations (from a
-         * non finite input: the reference would not return). */
+        /* A rejection or bisection loop reached 2^25 iterations (a non
+         * finite input, or kinetic energies far above the tables, where
+         * the reference would not return or needs more trials). */
         PUMAS_GPU_HISTORY_LOOP_LIMIT = 101,
         /* The target selection of a discrete energy loss matched no target
          * (the reference aborts on assert(0), src/pumas.c:6207, 6264). */
@@ -358,6 +359,9 @@ struct pumas_gpu_stats {
         uint64_t step_limited;  /* PUMAS_GPU_HISTORY_STEP_LIMIT */
         uint64_t loop_limited;  /* PUMAS_GPU_HISTORY_LOOP_LIMIT */
         uint64_t no_target;     /* PUMAS_GPU_HISTORY_NO_TARGET */
+        double init_kernel_ms;  /* time in the entry kernel (one lane per
+                                   history; the whole transport of the
+                                   closed form CSDA) */
 };
 
 /*

@@ -487,6 +487,35 @@ double ref_dcs_exact(const void * physics, int ip, int iel, double K, double q)
 double ref_dilog(double x) { return math_dilog(x); }
 
 /*
+ * Polar angle of a DEL (polar_get, src/pumas.c:11736-11976) for process `ip`
+ * on element `iel`, sample i with kinetic energies ki[i] -> kf[i] drawing
+ * from the transport Philox stream of history i under `seed`; the number of
+ * draws of sample i goes to draws[i]. Returns 0, or -1 without a context.
+ */
+int ref_polar(const void * physics, int ip, int iel, const double * ki,
+    const double * kf, int64_t n, uint64_t seed, double * mu,
+    uint32_t * draws)
+{
+        struct pumas_context * context = NULL;
+        if (pumas_context_create(&context, physics,
+                sizeof(struct ref_user)) != PUMAS_RETURN_SUCCESS)
+                return -1;
+        context->random = &ref_random;
+        struct ref_user * u = context->user_data;
+        const struct pumas_physics * p = physics;
+        polar_function_t * polar = polar_get(ip);
+        u->seed = seed;
+        for (int64_t i = 0; i < n; i++) {
+                u->history = (uint64_t)i;
+                u->draws = 0;
+                mu[i] = polar(p, context, p->element[iel], ki[i], kf[i]);
+                draws[i] = u->draws;
+        }
+        pumas_context_destroy(&context);
+        return 0;
+}
+
+/*
  * Transport `n` histories on one thread with the reference's default
  * recorder at `period`, writing up to `capacity` frames (state, medium index,
  * event, history id) in the order recorded. Returns the number of frames

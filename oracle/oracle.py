@@ -75,6 +75,9 @@ def library():
         lib.ref_dcs_exact.restype = ctypes.c_double
         lib.ref_dilog.argtypes = [ctypes.c_double]
         lib.ref_dilog.restype = ctypes.c_double
+        lib.ref_polar.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp,
+                                  ctypes.c_int64, ctypes.c_uint64, vp, vp]
+        lib.ref_polar.restype = ctypes.c_int
         _libs["ref"] = lib
     return _libs["ref"]
 
@@ -107,6 +110,19 @@ class RefPhysics:
 
     def dcs_exact(self, process, element, K, q):
         return self.lib.ref_dcs_exact(self.handle, process, element, K, q)
+
+    def polar(self, process, element, ki, kf, seed=0):
+        """Polar angles mu of DELs ki[i] -> kf[i] (sample i draws from the
+        transport stream of history i) and the draws of each sample."""
+        ki = np.ascontiguousarray(ki, np.float64)
+        kf = np.ascontiguousarray(kf, np.float64)
+        mu = np.zeros(len(ki))
+        draws = np.zeros(len(ki), np.uint32)
+        assert self.lib.ref_polar(self.handle, process, element,
+                                  ki.ctypes.data, kf.ctypes.data, len(ki),
+                                  seed, mu.ctypes.data,
+                                  draws.ctypes.data) == 0
+        return mu, draws
 
     def close(self):
         if self.handle:

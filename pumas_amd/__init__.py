@@ -290,13 +290,15 @@ class TransportStats:
     step_limited: int = 0
     loop_limited: int = 0
     no_target: int = 0
+    init_kernel_ms: float = 0.0
 
     @classmethod
     def from_c(cls, st):
         return cls(int(st.steps), int(st.histories), float(st.kernel_ms),
                    float(st.step_kernel_ms), int(st.step_launches),
                    int(st.rounds), int(st.lds_bytes), int(st.step_limited),
-                   int(st.loop_limited), int(st.no_target))
+                   int(st.loop_limited), int(st.no_target),
+                   float(st.init_kernel_ms))
 
 
 def _options(precision, block_size, blocks_per_cu, max_steps,

@@ -168,7 +168,8 @@ class GpuStats(ctypes.Structure):
                 ("rounds", ctypes.c_uint64), ("lds_bytes", ctypes.c_uint64),
                 ("step_limited", ctypes.c_uint64),
                 ("loop_limited", ctypes.c_uint64),
-                ("no_target", ctypes.c_uint64)]
+                ("no_target", ctypes.c_uint64),
+                ("init_kernel_ms", ctypes.c_double)]
 
 
 # Histories per call of pumas_gpu_transport_noalloc

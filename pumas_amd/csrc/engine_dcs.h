@@ -123,8 +123,8 @@ PG_FN void dcs_pair_production_range(const ModelView & mv, int iel, double m,
  * Real dilogarithm by the Chebyshev expansion of CERNLIB RDILOG (C332), the
  * method of math_dilog, src/pumas.c:12609-12686, with the same argument
  * transformations and coefficients so that the pair production DDCS rounds
- * identically (CERN Program Library, http://cds.cern.ch/record/2050865).
- * Only the branches reached by 0 < x < 1 are kept.
+ * identically (CERN Program Library, http://cds.cern.ch/record/2050865),
+ * every branch of math_dilog, src/pumas.c:12609-12686.
  */
 PG_FN double dilog(double x)
 {
@@ -138,19 +138,46 @@ PG_FN double dilog(double x)
                 -0.00000000000027007, 0.00000000000004042,
                 -0.00000000000000610, 0.00000000000000093,
                 -0.00000000000000014, 0.00000000000000002 };
+        const double PI3 = PI * PI / 3.;
         const double PI6 = PI * PI / 6.;
+        const double PI12 = PI * PI / 12.;
+        /* The reference's special values, kept as they are (x = 10, and
+         * -pi^2/12 at both x = -1 and x = 1). */
+        if (x == 10) return PI6;
+        if ((x == -1.) || (1. - x == 0.)) return -PI12;
         const double T = -x;
         double A, S, Y;
-        if (T <= -0.5) {
+        /* The reference squares with pow(y, 2.), which its compiler
+         * evaluates as y * y. */
+        if (T <= -2) {
+                Y = -1. / (1. + T);
+                S = 1.;
+                const double l0 = lm::log(-T), l1 = lm::log(1. + 1. / T);
+                A = -PI3 + 0.5 * (l0 * l0 - l1 * l1);
+        } else if (T < -1.) {
+                Y = -1. - T;
+                S = -1.;
+                A = lm::log(-T);
+                A = -PI6 + A * (A + lm::log(1. + 1. / T));
+        } else if (T <= -0.5) {
                 Y = -(1. + T) / T;
                 S = 1.;
                 A = lm::log(-T);
                 A = -PI6 + A * (-0.5 * A + lm::log(1. + T));
-        } else {
+        } else if (T < 0.) {
                 Y = -T / (1. + T);
                 S = -1.;
                 const double l = lm::log(1 + T);
                 A = 0.5 * (l * l);
+        } else if (T <= 1.) {
+                Y = T;
+                S = 1.;
+                A = 0.;
+        } else {
+                Y = 1. / T;
+                S = -1.;
+                const double l = lm::log(T);
+                A = PI6 + 0.5 * (l * l);
         }
         const double H = Y + Y - 1;
         const double ALFA = H + H;

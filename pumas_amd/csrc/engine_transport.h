@@ -86,10 +86,14 @@ struct Recorder {
 
 /*
  * The rejection and bisection loops of the reference have no iteration
- * limit. For finite inputs each of them accepts with a probability bounded
- * away from zero (at most 120 iterations over 10^7 voxel histories), but a
- * NaN or infinite input can make one never accept, and a single such lane
- * would keep its wavefront, and so the launch, from ever ending. After
+ * limit. For finite inputs within the tables each of them accepts with a
+ * probability bounded away from zero (at most 120 iterations over 10^7
+ * voxel histories). Above the tables (kinetic energies beyond 1e9 GeV) the
+ * elastic hard scattering angle loop accepts ever more rarely (the nuclear
+ * form factor cuts the large angles): the reference needs up to 3e7 trials
+ * around 1e10 GeV, and at 1e10 GeV forward one of its loops never ends; a
+ * NaN or infinite input can also make a loop never accept. A single such
+ * lane would keep its wavefront, and so the launch, from ending. After
  * LOOP_MAX iterations a loop gives up: its last values are used once and
  * the history is marked (max_steps = MARK_LOOP_LIMIT) so that it ends at
  * its next step boundary with rc HISTORY_LOOP_LIMIT and no event.
@@ -97,7 +101,7 @@ struct Recorder {
  * the same way (MARK_NO_TARGET, rc HISTORY_NO_TARGET).
  */
 #ifndef PG_LOOP_MAX
-#define PG_LOOP_MAX (1 << 16)
+#define PG_LOOP_MAX (1 << 25)
 #endif
 constexpr int LOOP_MAX = PG_LOOP_MAX;
 constexpr int MARK_LOOP_LIMIT = -1;
@@ -1716,6 +1720,7 @@ struct EngineT {
          * only in the entry kernel (one lane per history, not persistent)
          * and is inlined there.
          */
+        template <bool CLOSED = false>
         PG_FN void walker_init(Walker & w) const
         {
                 Track & t = w.t;
@@ -1786,7 +1791,11 @@ struct EngineT {
                 /* The end medium of a history that stops before stepping
                  * is its start medium. */
                 w.end_medium = medium;
-                if (!do_stepping) {
+                /* CLOSED: a launch for contexts and geometries where every
+                 * history takes the closed form (CSDA, no scattering, a
+                 * geometry without step limits), without the stepping set
+                 * up below. */
+                if (CLOSED || !do_stepping) {
                         record_frame(w, medium, EVENT_START);
                         w.event = transport_with_csda(t, material, density);
                         record_frame(w, medium, w.event | EVENT_STOP);

@@ -862,7 +862,7 @@ __device__ __forceinline__ void steps_flush(
  * 104 B strides. */
 constexpr int INIT_BLOCK = 256;
 
-template <bool F32>
+template <bool F32, bool CLOSED>
 __global__ void __launch_bounds__(INIT_BLOCK) k_init(KernelArgs args)
 {
         KERNEL_ARGS(args);
@@ -883,16 +883,17 @@ __global__ void __launch_bounds__(INIT_BLOCK) k_init(KernelArgs args)
         w.phase = PHASE_EMPTY;
         if (active) {
                 walker_load(a, i, w, stage + threadIdx.x);
-                eng.walker_init(w);
+                eng.template walker_init<CLOSED>(w);
                 if (w.phase == PHASE_DONE)
                         walker_store(a, i, w, stage + threadIdx.x);
-                if (w.phase == PHASE_STEP) a.records[i] = w;
+                if (!CLOSED && (w.phase == PHASE_STEP)) a.records[i] = w;
         }
         __syncthreads();
         for (uint32_t k = threadIdx.x; k < words; k += INIT_BLOCK)
                 hbm[k] = lds[k];
-        queue_append(a.counters + C_STEP_N0, a.q_step[0],
-            w.phase == PHASE_STEP, (uint32_t)i);
+        if (!CLOSED)
+                queue_append(a.counters + C_STEP_N0, a.q_step[0],
+                    w.phase == PHASE_STEP, (uint32_t)i);
 }
 
 /* Default maximum number of lanes of a wavefront waiting for an elastic
@@ -1110,6 +1111,8 @@ struct Launch {
         unsigned step_grid, del_grid;
         hipStream_t stream;
         hipEvent_t * marks; /* [2 * rounds_per_check] or NULL */
+        hipEvent_t * init_marks; /* [2], around the entry kernel, or NULL */
+        bool closed; /* every history takes the closed form CSDA */
 };
 
 template <bool F32, bool TAIL_DEL>
@@ -1247,8 +1250,11 @@ enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
         ChunkResources res;
         res.stream = stream;
         const size_t b_counters = 256;
-        const size_t b_records = ((m * sizeof(Walker)) + 255) & ~size_t(255);
-        const size_t b_queue = ((m * sizeof(uint32_t)) + 255) & ~size_t(255);
+        /* A closed form launch needs no records or queues. */
+        const size_t b_records = L.closed ? 0 :
+            ((m * sizeof(Walker)) + 255) & ~size_t(255);
+        const size_t b_queue = L.closed ? 0 :
+            ((m * sizeof(uint32_t)) + 255) & ~size_t(255);
         char * ptr;
         if (fixed) {
                 ptr = static_cast<char *>(fixed);
@@ -1269,12 +1275,20 @@ enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
         hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, stream,
             a.counters, C_COUNT);
         const unsigned blocks = (unsigned)((m + INIT_BLOCK - 1) / INIT_BLOCK);
-        if (f32)
-                hipLaunchKernelGGL(k_init<true>, dim3(blocks),
+        if (L.init_marks) HIP_TRY(hipEventRecord(L.init_marks[0], stream));
+        if (f32 && L.closed)
+                hipLaunchKernelGGL((k_init<true, true>), dim3(blocks),
+                    dim3(INIT_BLOCK), 0, stream, a);
+        else if (f32)
+                hipLaunchKernelGGL((k_init<true, false>), dim3(blocks),
+                    dim3(INIT_BLOCK), 0, stream, a);
+        else if (L.closed)
+                hipLaunchKernelGGL((k_init<false, true>), dim3(blocks),
                     dim3(INIT_BLOCK), 0, stream, a);
         else
-                hipLaunchKernelGGL(k_init<false>, dim3(blocks),
+                hipLaunchKernelGGL((k_init<false, false>), dim3(blocks),
                     dim3(INIT_BLOCK), 0, stream, a);
+        if (L.init_marks) HIP_TRY(hipEventRecord(L.init_marks[1], stream));
         if (hipGetLastError() != hipSuccess)
                 return pg_fail(PUMAS_RETURN_VALUE_ERROR,
                     "transport kernel launch failed");
@@ -1305,10 +1319,19 @@ enum pumas_return transport_chunk(const KernelArgs & a0, size_t base,
                 HIP_TRY(ei);
         }
         int parity = 0, launched = 0;
+        bool init_timed = false;
         for (;;) {
                 HIP_TRY(hipMemcpyAsync(host, a.counters, sizeof(host),
                     hipMemcpyDeviceToHost, stream));
                 HIP_TRY(hipStreamSynchronize(stream));
+                if (L.init_marks && stats && (parity == 0) &&
+                    (launched == 0) && !init_timed) {
+                        float ms = 0.f;
+                        HIP_TRY(hipEventElapsedTime(&ms, L.init_marks[0],
+                            L.init_marks[1]));
+                        stats->init_kernel_ms += ms;
+                        init_timed = true;
+                }
                 if (L.marks && stats) {
                         for (int r = 0; r < launched; r++) {
                                 float ms = 0.f;
@@ -1490,12 +1513,15 @@ static enum pumas_return transport_impl(const pumas_gpu_physics * physics,
         /* Timing events, destroyed on every return path. */
         struct Events {
                 hipEvent_t e0 = nullptr, e1 = nullptr;
+                hipEvent_t init[2] = { nullptr, nullptr };
                 std::vector<hipEvent_t> marks;
 
                 ~Events()
                 {
                         if (e0) (void)hipEventDestroy(e0);
                         if (e1) (void)hipEventDestroy(e1);
+                        for (auto & e : init)
+                                if (e) (void)hipEventDestroy(e);
                         for (auto & e : marks)
                                 if (e) (void)hipEventDestroy(e);
                 }
@@ -1503,6 +1529,8 @@ static enum pumas_return transport_impl(const pumas_gpu_physics * physics,
         if (stats && !fixed) {
                 HIP_TRY(hipEventCreate(&ev.e0));
                 HIP_TRY(hipEventCreate(&ev.e1));
+                HIP_TRY(hipEventCreate(&ev.init[0]));
+                HIP_TRY(hipEventCreate(&ev.init[1]));
                 HIP_TRY(hipEventRecord(ev.e0, stream));
         }
         const bool f32 = (precision == PUMAS_GPU_PRECISION_FP32_BULK);
@@ -1531,9 +1559,16 @@ static enum pumas_return transport_impl(const pumas_gpu_physics * physics,
                 return pg_fail(PUMAS_RETURN_VALUE_ERROR,
                     "lanes_per_wave must be at most 64 (got %d)",
                     lanes_per_wave);
+        /* Every history takes the closed form CSDA when the geometry has
+         * no step limits (infinite medium), the energy loss is CSDA and
+         * scattering is disabled (src/pumas.c:3200-3230). */
+        const bool closed = (geometry->view.kind == GEO_INFINITE) &&
+            (context->mode.energy_loss == PUMAS_MODE_CSDA) &&
+            (context->mode.scattering == PUMAS_MODE_DISABLED);
         const Launch L = { n_cu, block, per_cu, rounds_per_check, graph,
                 tail_del, lanes_per_wave, lds_bytes, 0, 0, stream,
-                (stats && !graph && !fixed) ? ev.marks.data() : nullptr };
+                (stats && !graph && !fixed) ? ev.marks.data() : nullptr,
+                (stats && !fixed) ? ev.init : nullptr, closed };
         for (size_t base = 0; base < n; base += MAX_CHUNK) {
                 const size_t m = (n - base < MAX_CHUNK) ? n - base : MAX_CHUNK;
                 rc = transport_chunk(

@@ -26,6 +26,62 @@ from pumas_amd import workloads as W  # noqa: E402
 SEED = 0
 
 
+def edge_states(n_each, energies, grammage=0.0, distance=0.0, time=0.0,
+                direction=-1.0, z=0.0):
+    """States in the cases of the reference's own CSDA tests
+    (tests/test-pumas.c:2361-3186): charge -1, weight 1, at z (m), moving
+    along +-z, each of `energies` repeated n_each times with the given
+    initial grammage, distance and time."""
+    e = np.repeat(np.asarray(energies, float), n_each)
+    s = np.zeros(len(e), W.STATE_DTYPE)
+    s["charge"] = -1.0
+    s["energy"] = e
+    s["weight"] = 1.0
+    s["grammage"] = grammage
+    s["distance"] = distance
+    s["time"] = time
+    s["position"][:, 2] = z
+    s["direction"][:, 2] = direction
+    return s
+
+
+def limit_violation_cases():
+    """Every limit type violated at the start, forward and backward, in the
+    closed form CSDA (tests/test-pumas.c:2916-3140) and in straggled
+    stepping; plus histories that start within the limits."""
+    rock = W.csda_rock_geometry()
+    base = dict(energy_loss=Mode.CSDA, decay=Mode.WEIGHTED,
+                scattering=Mode.DISABLED)
+    limits = dict(limit_energy=0.5, limit_grammage=2.0e4,
+                  limit_distance=10.0, limit_time=3e-8)
+    for direction, name in ((Mode.BACKWARD, "backward"),
+                            (Mode.FORWARD, "forward")):
+        for ename, event, field, inside, outside in (
+                ("energy", Event.LIMIT_ENERGY, "energy",
+                 (0.25, 0.2) if direction == Mode.BACKWARD else (1.0, 2.0),
+                 (1.0, 4.0) if direction == Mode.BACKWARD else (0.5, 0.25)),
+                ("grammage", Event.LIMIT_GRAMMAGE, "grammage", (0.0, 1e4),
+                 (2.0e4, 3.0e4)),
+                ("distance", Event.LIMIT_DISTANCE, "distance", (0.0, 5.0),
+                 (10.0, 20.0)),
+                ("time", Event.LIMIT_TIME, "time", (0.0, 1e-8),
+                 (3e-8, 1e-7))):
+            for mode, geom, tag in (
+                    (dict(base), rock, "csda"),
+                    (dict(energy_loss=Mode.STRAGGLED, decay=Mode.WEIGHTED,
+                          scattering=Mode.MIXED), rock, "straggled")):
+                ctx = W.Context(direction=direction, event=event, **mode,
+                                **limits)
+                rows = []
+                for v in inside + outside:
+                    kw = {"energies": [1.0 if ename != "energy" else v]}
+                    if ename != "energy":
+                        kw[field] = v
+                    rows.append(edge_states(4, direction=1.0, **kw))
+                yield (f"violation_{tag}_{ename}_{name}", "standard_c005",
+                       ctx, geom, np.concatenate(rows))
+
+
 def cases():
     """(name, physics, context, geometry description, initial states)."""
     yield ("csda_rock", "standard_c005", W.csda_rock_context(),
@@ -53,11 +109,36 @@ def cases():
                event=Event.LIMIT_TIME | Event.LIMIT_ENERGY,
                limit_time=300.0, limit_energy=2e3),
            W.slabs_geometry(), W.slabs_backward_states(128, seed=SEED))
+    # Kinetic energies above the last table row (K_max = 1e9 GeV): the
+    # constant-loss extrapolation of the tables (tests/test-pumas.c:
+    # 3142-3186), closed form and stepping.
+    over = [2e9, 1e10, 1e11, 1e9, 5e8]
+    yield ("overflow_csda_forward", "standard_c005",
+           W.Context(energy_loss=Mode.CSDA, decay=Mode.WEIGHTED,
+                     direction=Mode.FORWARD, scattering=Mode.DISABLED),
+           W.csda_rock_geometry(), edge_states(8, over, direction=1.0))
+    # Stepping above K_max: up to 2e9 GeV (from ~1e10 GeV the reference's
+    # elastic hard scattering angle loop needs up to millions of trials,
+    # beyond 2^25 at 3e10 GeV, where the engine ends the history with
+    # PUMAS_GPU_HISTORY_LOOP_LIMIT).
+    yield ("overflow_straggled_forward", "standard_c005",
+           W.slabs_forward_context(), W.slabs_geometry(),
+           edge_states(8, [2e9, 1.5e9, 1e9, 5e8], direction=-1.0,
+                       z=W.SLAB_START_Z))
+    # Backward from 1e8-6e8 GeV the energy grows past K_max (to ~1e10 GeV)
+    # within the stack.
+    yield ("overflow_straggled_backward", "standard_c005",
+           W.slabs_backward_context(), W.slabs_geometry(),
+           edge_states(8, [1e8, 2e8, 4e8, 6e8], direction=-1.0, z=1e-3))
+    yield from limit_violation_cases()
 
 
-def main():
+def main(only=None):
+    """Writes the fixtures of `only` (names), or of every case."""
     physics = {}
     for name, pname, ctx, desc, states in cases():
+        if only and name not in only:
+            continue
         if pname not in physics:
             physics[pname] = orc.RefPhysics(pname)
         geom = orc.RefGeometry(desc)
@@ -75,4 +156,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

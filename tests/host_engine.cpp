@@ -143,3 +143,51 @@ extern "C" long host_check_k_index(long n_random) {
     }
     return bad;
 }
+
+/* Function-level entry points on the loaded tables (see probe.hip for the
+ * device build of the same routines): which 0 exact DCS, 1 tabulated DCS,
+ * 2 dilog of a[i], 3 polar angle of a DEL with the random stream of history
+ * i under `seed` (draws of sample i in draws[i]). */
+extern "C" void host_functions(int which, int ip, int iel, const double *a,
+    const double *b, double *out, uint32_t *draws, uint64_t seed, int64_t n) {
+    for (int64_t i = 0; i < n; i++) {
+        if (which == 0) out[i] = dcs_exact(g_view, g_mv, ip, iel, a[i], b[i]);
+        else if (which == 1) { Cache c; c.reset(g_view.n_energies);
+            out[i] = dcs_evaluate(g_view, g_mv, c, ip, iel, a[i], b[i]); }
+        else if (which == 2) out[i] = dilog(a[i]);
+        else { Rng rng; rng.init(seed, (uint64_t)i, STREAM_TRANSPORT, 0);
+            out[i] = polar_sample(g_view, rng, ip, iel, a[i], b[i]);
+            if (draws) draws[i] = rng.count; }
+    }
+}
+
+/* The table accessors on the loaded tables, numbered as the oracle's
+ * ref_cel: 0 cel_grammage, 1 cel_proper_time, 2 cel_kinetic_energy,
+ * 3 cel_energy_loss, 4 cel_straggling, 5 cel_grammage_as_time,
+ * 6 del_cross_section, 7 del_interaction_length,
+ * 8 del_kinetic_from_interaction_length, 9 ehs_interaction_length,
+ * 10 ehs_kinetic_from_interaction_length, 11 coulomb_ehs_length.
+ * Each value uses fresh table index caches, as the oracle's calls. */
+extern "C" void host_cel(int which, int scheme, int material, const double *x,
+    double *out, int64_t n) {
+    const PhysicsView &p = g_view;
+    for (int64_t i = 0; i < n; i++) {
+        Cache c; c.reset(p.n_energies);
+        const double v = x[i];
+        switch (which) {
+        case 0: out[i] = cel_grammage(p, c, scheme, material, v); break;
+        case 1: out[i] = cel_proper_time(p, c, scheme, material, v); break;
+        case 2: out[i] = cel_kinetic_energy(p, c, scheme, material, v); break;
+        case 3: out[i] = cel_energy_loss(p, c, scheme, material, v); break;
+        case 4: out[i] = cel_straggling(p, c, material, v); break;
+        case 5: out[i] = cel_grammage_as_time(p, c, scheme, material, v); break;
+        case 6: out[i] = del_cross_section(p, c, material, v); break;
+        case 7: out[i] = del_interaction_length(p, c, material, v); break;
+        case 8: out[i] = del_kinetic_from_interaction_length(p, c, material, v); break;
+        case 9: out[i] = ehs_interaction_length(p, c, scheme, material, v); break;
+        case 10: out[i] = ehs_kinetic_from_interaction_length(p, c, scheme, material, v); break;
+        case 11: out[i] = coulomb_ehs_length(p, c, material, v); break;
+        default: out[i] = 0.;
+        }
+    }
+}

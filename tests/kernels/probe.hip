@@ -8,7 +8,12 @@
 #include <stdint.h>
 #include <stdio.h>
 
-#include "libm.h"
+#include <string.h>
+#include <vector>
+
+#include "../../include/pumas_gpu.h"
+#include "engine_transport.h"
+#include "ptab.h"
 
 enum {
         F_EXP = 0, F_LOG, F_POW, F_SIN, F_COS, F_SINCOS_S, F_SINCOS_C, F_LOG10,
@@ -104,4 +109,158 @@ extern "C" void probe_host_math(int which, const double * x, const double * y,
     double * out, size_t n)
 {
         for (size_t i = 0; i < n; i++) out[i] = host_value(which, x[i], y[i]);
+}
+
+/* ------------------------------------------------------------------------
+ * Engine routines on a PTAB image copied to the device as is: the views
+ * point at its sections (the layout the library uploads).
+ * ------------------------------------------------------------------------ */
+struct DeviceTables {
+        std::vector<uint8_t> image;
+        uint8_t * d_image = nullptr;
+        int16_t * d_coarse = nullptr;
+        pg::PhysicsView view;
+        pg::ModelView mv;
+};
+static DeviceTables g_t;
+
+static const void * section(const char * name, bool device)
+{
+        const ptab_header * h = (const ptab_header *)g_t.image.data();
+        const ptab_entry * e =
+            (const ptab_entry *)(g_t.image.data() + sizeof(ptab_header));
+        for (uint32_t i = 0; i < h->n_entries; i++)
+                if (!strncmp(e[i].name, name, PTAB_NAME_LEN))
+                        return (device ? g_t.d_image : g_t.image.data()) +
+                            e[i].offset;
+        fprintf(stderr, "probe: missing section %s\n", name);
+        return nullptr;
+}
+
+/* Loads the PTAB file at `path` for the probes below. */
+extern "C" int probe_load(const char * path)
+{
+        FILE * f = fopen(path, "rb");
+        if (!f) return -1;
+        fseek(f, 0, SEEK_END);
+        const long n = ftell(f);
+        fseek(f, 0, SEEK_SET);
+        g_t.image.resize(n);
+        if (fread(g_t.image.data(), 1, n, f) != (size_t)n) return -1;
+        fclose(f);
+        if (g_t.d_image) (void)hipFree(g_t.d_image);
+        if (g_t.d_coarse) (void)hipFree(g_t.d_coarse);
+        TRY(hipMalloc(&g_t.d_image, n));
+        TRY(hipMemcpy(g_t.d_image, g_t.image.data(), n, hipMemcpyHostToDevice));
+        const ptab_header * h = (const ptab_header *)g_t.image.data();
+        pg::PhysicsView & v = g_t.view;
+        memset(&v, 0, sizeof(v));
+        v.n_energies = h->imeta[PTAB_N_ENERGIES];
+        v.n_materials = h->imeta[PTAB_N_MATERIALS];
+        v.n_elements = h->imeta[PTAB_N_ELEMENTS];
+        v.n_components = h->imeta[PTAB_N_COMPONENTS];
+        v.n_table_dcs = h->imeta[PTAB_N_TABLE_DCS];
+        v.max_components = h->imeta[PTAB_MAX_COMPONENTS];
+        v.mass = h->fmeta[PTAB_MASS];
+        v.ctau = h->fmeta[PTAB_CTAU];
+        v.cutoff = h->fmeta[PTAB_CUTOFF];
+#define D(f, s) v.f = (const double *)section(s, true)
+        D(K, "K"); D(K_dX, "K_dX"); D(K_dNI_in, "K_dNI_in");
+        D(K_dNI_el, "K_dNI_el"); D(X, "X"); D(X_dK, "X_dK"); D(X_dT, "X_dT");
+        D(T, "T"); D(T_dK, "T_dK"); D(dE, "dE"); D(dE_dK, "dE_dK");
+        D(Omega, "Omega"); D(Omega_dK, "Omega_dK"); D(NI_el, "NI_el");
+        D(NI_el_dK, "NI_el_dK"); D(NI_in, "NI_in"); D(NI_in_dK, "NI_in_dK");
+        D(CS, "CS"); D(CS_dK, "CS_dK"); D(CSf, "CSf"); D(CSf_dK, "CSf_dK");
+        D(CSn, "CSn"); D(CSn_dK, "CSn_dK"); D(Kt, "Kt"); D(a_max, "a_max");
+        D(b_max, "b_max"); D(Mu0, "Mu0"); D(Mu0_dK, "Mu0_dK"); D(Lb, "Lb");
+        D(Lb_dK, "Lb_dK"); D(Ms1, "Ms1"); D(Ms1_dK, "Ms1_dK");
+        D(material_density, "material_density");
+        D(component_fraction, "component_fraction");
+        D(element_Z, "element_Z"); D(element_A, "element_A");
+        D(element_I, "element_I"); D(element_nscr, "element_n_screening");
+        D(element_amplitude, "element_amplitude");
+        D(element_exponent, "element_exponent");
+        D(element_rn, "element_nuclear_radius");
+        D(element_lrad, "element_radiation_log");
+#undef D
+        v.DCS = (const float *)section("DCS", true);
+        v.DCS_x = (const float *)section("DCS_x", true);
+        v.DCS_envelope = (const float *)section("DCS_envelope", true);
+        v.elements_in = (const int *)section("elements_in", true);
+        v.component_element = (const int *)section("component_element", true);
+        v.element_index = (const int *)section("element_index", true);
+        v.material_component0 = nullptr;
+        std::vector<int16_t> coarse(4096, 0);
+        v.k_nbins = pg::k_coarse_build((const double *)section("K", false),
+            v.n_energies, coarse.data(), (int)coarse.size(), &v.k_bin0);
+        TRY(hipMalloc(&g_t.d_coarse, coarse.size() * sizeof(int16_t)));
+        TRY(hipMemcpy(g_t.d_coarse, coarse.data(),
+            coarse.size() * sizeof(int16_t), hipMemcpyHostToDevice));
+        v.k_coarse = v.k_nbins ? g_t.d_coarse : nullptr;
+        pg::ModelView & m = g_t.mv;
+        m.z13 = (const double *)section("element_z13", true);
+        m.zm13 = (const double *)section("element_zm13", true);
+        m.z23 = (const double *)section("element_z23", true);
+        m.dn = (const double *)section("element_dn", true);
+        m.lrad = v.element_lrad;
+        m.cst = (const double *)section("constants", true);
+        m.gq9_x = (const double *)section("gq9_x", true);
+        m.gq9_w = (const double *)section("gq9_w", true);
+        m.gq12_x = (const double *)section("gq12_x", true);
+        m.gq12_w = (const double *)section("gq12_w", true);
+        return 0;
+}
+
+/* which: 0 exact DCS (dcs_exact), 1 tabulated DCS (dcs_evaluate), 2 dilog
+ * of a[i], 3 polar angle of a DEL (random stream (seed, i)), draws in
+ * draws[i]. */
+__global__ void k_functions(pg::PhysicsView p, pg::ModelView mv, int which,
+    int ip, int iel, const double * a, const double * b, double * out,
+    uint32_t * draws, uint64_t seed, size_t n)
+{
+        const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= n) return;
+        if (which == 0) {
+                out[i] = pg::dcs_exact(p, mv, ip, iel, a[i], b[i]);
+        } else if (which == 1) {
+                pg::Cache c;
+                c.reset(p.n_energies);
+                out[i] = pg::dcs_evaluate(p, mv, c, ip, iel, a[i], b[i]);
+        } else if (which == 2) {
+                out[i] = pg::dilog(a[i]);
+        } else {
+                pg::Rng rng;
+                rng.init(seed, i, pg::STREAM_TRANSPORT, 0);
+                out[i] = pg::polar_sample(p, rng, ip, iel, a[i], b[i]);
+                draws[i] = rng.count;
+        }
+}
+
+extern "C" int probe_functions(int which, int ip, int iel, const double * a,
+    const double * b, double * out, uint32_t * draws, uint64_t seed,
+    size_t n)
+{
+        double *da = nullptr, *db = nullptr, *dout = nullptr;
+        uint32_t * ddr = nullptr;
+        const size_t bytes = n * sizeof(double);
+        TRY(hipMalloc(&da, bytes));
+        TRY(hipMalloc(&db, bytes));
+        TRY(hipMalloc(&dout, bytes));
+        TRY(hipMalloc(&ddr, n * sizeof(uint32_t)));
+        TRY(hipMemcpy(da, a, bytes, hipMemcpyHostToDevice));
+        TRY(hipMemcpy(db, b, bytes, hipMemcpyHostToDevice));
+        const unsigned block = 64;
+        hipLaunchKernelGGL(k_functions, dim3((unsigned)((n + block - 1) / block)),
+            dim3(block), 0, 0, g_t.view, g_t.mv, which, ip, iel, da, db, dout,
+            ddr, seed, n);
+        TRY(hipGetLastError());
+        TRY(hipMemcpy(out, dout, bytes, hipMemcpyDeviceToHost));
+        if (draws)
+                TRY(hipMemcpy(draws, ddr, n * sizeof(uint32_t),
+                    hipMemcpyDeviceToHost));
+        TRY(hipFree(da));
+        TRY(hipFree(db));
+        TRY(hipFree(dout));
+        TRY(hipFree(ddr));
+        return 0;
 }

@@ -251,3 +251,17 @@ def test_loop_limit_status():
     assert (out[1][flagged] == int(pa.Event.NONE)).all()
     assert (out[5][~flagged] == 0).all()
     assert (out[0][~flagged] == ok[0][~flagged]).all()
+
+
+def test_energy_above_tables_loop_limit(host):
+    """Forward straggled transport at 1e11 GeV, a hundred times the last
+    table row: the elastic hard scattering angle loop of the reference
+    accepts so rarely there (the nuclear form factor cuts the large angles)
+    that it needs more than 2^25 trials. The engine ends the history after
+    that many with PUMAS_GPU_HISTORY_LOOP_LIMIT and no event."""
+    import make_golden as M
+    s = M.edge_states(2, [1e11], direction=-1.0, z=W.SLAB_START_Z)
+    out = run_host(host, "standard_c005", W.slabs_forward_context(),
+                   W.slabs_geometry(), s)
+    assert (out[5] == int(pa.HistoryStatus.LOOP_LIMIT)).all()
+    assert (out[1] == int(pa.Event.NONE)).all()
