@@ -92,6 +92,10 @@ def parse():
                          "when they fit, -1 never")
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--ehs-batch", type=int, default=0,
+                    help="lanes waiting for an elastic hard scattering "
+                         "before the wavefront runs them (0: default or "
+                         "tuned)")
     ap.add_argument("--serve-seconds", type=float, default=0.25,
                     help="arrival window of the serving run (Poisson "
                          "requests at 200/s); 0 skips it")
@@ -376,9 +380,10 @@ def main():
         pa.Precision.FP32_BULK
 
     tuned = {"block_size": args.block_size,
-             "blocks_per_cu": args.blocks_per_cu, "ehs_batch": 0}
+             "blocks_per_cu": args.blocks_per_cu,
+             "ehs_batch": args.ehs_batch}
     if (not args.no_tune and not csda and args.block_size == 0 and
-            args.blocks_per_cu == 0):
+            args.blocks_per_cu == 0 and args.ehs_batch == 0):
         m = min(n, args.tune_n)
         tuned, rate = pa.autotune(phys, ctx, geom, pristine.data_ptr(), m,
                                   seed=7, precision=precision, stream=stream)

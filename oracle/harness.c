@@ -139,6 +139,7 @@ struct ref_geometry {
         int64_t n_media;
         int * material;
         double * density;
+        int lazy;
 };
 
 /* Uniform medium with an explicit density (`pumas_locals_cb`). */
@@ -148,6 +149,32 @@ static double ref_locals(struct pumas_medium * medium,
         (void)state;
         locals->density = ((struct ref_medium *)medium)->density;
         return 0.;
+}
+
+/* The reference medium of device medium index k (its material, and a
+ * locals callback with its density when that is set). The density is
+ * stored last with release semantics: a medium filled by two threads at
+ * once gets the same values twice. */
+static void ref_medium_fill(struct ref_geometry * r, int64_t k)
+{
+        if ((r->g.kind == GEO_VOXELS) && (k == 0)) return;
+        int material;
+        double density;
+        geo_medium_properties(&r->g, (int)k, &material, &density);
+        r->media[k].api.material = material;
+        r->media[k].api.locals = (density > 0.) ? &ref_locals : NULL;
+        __atomic_store(&r->media[k].density, &density, __ATOMIC_RELEASE);
+}
+
+static struct pumas_medium * ref_medium_at(struct ref_geometry * r, int m)
+{
+        if (m < 0) return NULL;
+        if (r->lazy) {
+                double density;
+                __atomic_load(&r->media[m].density, &density, __ATOMIC_ACQUIRE);
+                if (density == 0.) ref_medium_fill(r, m);
+        }
+        return &r->media[m].api;
 }
 
 void * ref_geometry_create(const struct pumas_gpu_geometry_desc * d)
@@ -192,15 +219,13 @@ void * ref_geometry_create(const struct pumas_gpu_geometry_desc * d)
                 n = d->n_media;
         r->n_media = n;
         r->media = calloc(n, sizeof(struct ref_medium));
-        int64_t k;
-        for (k = 0; k < n; k++) {
-                if ((d->kind == GEO_VOXELS) && (k == 0)) continue;
-                int material;
-                double density;
-                geo_medium_properties(g, (int)k, &material, &density);
-                r->media[k].api.material = material;
-                r->media[k].density = density;
-                r->media[k].api.locals = (density > 0.) ? &ref_locals : NULL;
+        /* Grid media (one per terrain column or voxel, up to 1.3e8 of them
+         * for 512^3 voxels) are filled on first use (ref_medium_at); the
+         * untouched part of the calloc'ed table is never mapped. */
+        r->lazy = (d->kind == GEO_TERRAIN) || (d->kind == GEO_VOXELS);
+        if (!r->lazy) {
+                int64_t k;
+                for (k = 0; k < n; k++) ref_medium_fill(r, k);
         }
         return r;
 }
@@ -234,7 +259,7 @@ static enum pumas_step ref_medium_cb(struct pumas_context * context,
     struct pumas_state * state, struct pumas_medium ** medium, double * step)
 {
         struct ref_user * u = context->user_data;
-        const struct ref_geometry * r = u->geometry;
+        struct ref_geometry * r = (struct ref_geometry *)u->geometry;
         const double sgn =
             (context->mode.direction == PUMAS_MODE_FORWARD) ? 1. : -1.;
         int m;
@@ -245,7 +270,7 @@ static enum pumas_step ref_medium_cb(struct pumas_context * context,
                 *step = s;
                 u->steps++;
         }
-        if (medium != NULL) *medium = (m < 0) ? NULL : &r->media[m].api;
+        if (medium != NULL) *medium = ref_medium_at(r, m);
         return (type == GEO_STEP_RAW) ? PUMAS_STEP_RAW : PUMAS_STEP_CHECK;
 }
 
