@@ -497,14 +497,17 @@ def main():
         graph_replay["vs_eager"] = graph_replay["value"] / steps_per_s
     achieved = total_steps * unit_bytes / (kernel_ms * 1e-3) / 1e9
 
-    # HBM bytes per launch of the stepping kernel, from the committed
-    # FETCH_SIZE / WRITE_SIZE profile of this workload, used only when the
-    # profile was taken with the launch settings timed here.
+    # HBM bytes per launch of the dominant kernel, from the committed
+    # FETCH_SIZE / WRITE_SIZE profile of this workload at its default size,
+    # used only when the profile was taken with the launch settings timed
+    # here.
     traffic = None
     traffic_note = None
-    tpath = os.path.join(HERE, "profiles", "k_step_slabs_traffic.json")
-    if (args.workload == "slabs" and os.path.exists(tpath) and
-            n == 10_000_000 and args.precision == "fp64"):
+    profiled = {"slabs": 10_000_000, "csda": 100_000_000}
+    tpath = os.path.join(HERE, "profiles",
+                         f"{kernel}_{args.workload}_traffic.json")
+    if (profiled.get(args.workload) == n and os.path.exists(tpath) and
+            args.precision == "fp64"):
         with open(tpath) as f:
             prof = json.load(f)
         if prof.get("launch") == tuned:
@@ -571,7 +574,8 @@ def main():
             "bytes_per_unit": unit_bytes,
             "traffic_note": traffic_note,
             "measured_bytes_per_unit": (
-                traffic * launches / total_steps if traffic else None),
+                traffic * launches / (n * args.steps if csda else
+                                      total_steps) if traffic else None),
         },
         "transport_ms_per_pass": mean_ms,
         "history_status": {
