@@ -198,8 +198,9 @@ double pumas_gpu_physics_cutoff(const pumas_gpu_physics * physics);
 /* ------------------------------------------------------------------------
  * Geometry: device media replacing the `pumas_medium_cb` callback
  * (include/pumas.h:509-511) and the `pumas_locals_cb` callback
- * (include/pumas.h:348-349). A medium is a material index and a uniform
- * density; media are identified by index (-1 = outside, i.e. a NULL medium).
+ * (include/pumas.h:348-349). A medium is a material index and its local
+ * properties, uniform or an exponential density profile; media are
+ * identified by index (-1 = outside, i.e. a NULL medium).
  * ------------------------------------------------------------------------ */
 enum pumas_gpu_geometry_kind {
         /* One infinite uniform medium: `medium[0]`, no boundary. */
@@ -213,10 +214,33 @@ enum pumas_gpu_geometry_kind {
         PUMAS_GPU_GEOMETRY_VOXELS = 3
 };
 
+/* Local properties of a medium: what its `pumas_locals_cb` returns. */
+enum pumas_gpu_locals {
+        /* Uniform density, a null proposed step (a uniform medium). */
+        PUMAS_GPU_LOCALS_UNIFORM = 0,
+        /* Density profile along z, as the non uniform atmospheres of the
+         * reference's examples and tests (examples/pumas/geometry.c:78-97,
+         * tests/test-pumas.c:1871-1884):
+         *   density = density * exp(-(z - z_ref) / scale)
+         *   step    = scale / max(|u_z|, cos_min)
+         * with u the direction of the state. The reference multiplies the
+         * returned step by the context accuracy (src/pumas.c:5528-5552) and
+         * limits every Monte Carlo step to it. cos_min = 1 proposes `scale`
+         * in every direction. Allowed for the media of INFINITE and SLABS
+         * geometries and for the air (media[0]) of TERRAIN. */
+        PUMAS_GPU_LOCALS_EXPONENTIAL = 1
+};
+
 struct pumas_gpu_medium {
         int material;
-        /* Density in kg/m^3; zero or less selects the material default. */
+        /* Density in kg/m^3; zero or less selects the material default.
+         * EXPONENTIAL: the density at z = z_ref. */
         double density;
+        /* Fields below: zero for a uniform medium. */
+        enum pumas_gpu_locals locals;
+        double scale;   /* EXPONENTIAL: scale length, in m (> 0) */
+        double z_ref;   /* EXPONENTIAL: reference altitude, in m */
+        double cos_min; /* EXPONENTIAL: in (0, 1] */
 };
 
 struct pumas_gpu_geometry_desc {
@@ -589,6 +613,11 @@ void pumas_gpu_server_destroy(pumas_gpu_server ** server);
 /* Queue a request; returns its ticket. The request structure is copied. */
 enum pumas_return pumas_gpu_server_submit(pumas_gpu_server * server,
     const struct pumas_gpu_request * request, uint64_t * ticket);
+
+/* Device bytes that a request of `n` histories holds while in flight, the
+ * share of the HBM budget its admission needs (`device`: as the request's
+ * field, states and outputs already in device memory). */
+size_t pumas_gpu_server_request_bytes(size_t n, int device);
 
 /* 1 if the request is done, 0 if pending, -1 for an unknown ticket. */
 int pumas_gpu_server_poll(pumas_gpu_server * server, uint64_t ticket);

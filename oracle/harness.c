@@ -131,6 +131,9 @@ int ref_physics_export(void * physics, const char * path)
 struct ref_medium {
         struct pumas_medium api;
         double density;
+        /* Exponential profile (pumas_gpu_medium): scale length, reference
+         * altitude and minimum direction cosine. */
+        double scale, z_ref, cos_min;
 };
 
 struct ref_geometry {
@@ -139,6 +142,7 @@ struct ref_geometry {
         int64_t n_media;
         int * material;
         double * density;
+        const struct pumas_gpu_medium * desc_media; /* copy of the table */
         int lazy;
 };
 
@@ -149,6 +153,28 @@ static double ref_locals(struct pumas_medium * medium,
         (void)state;
         locals->density = ((struct ref_medium *)medium)->density;
         return 0.;
+}
+
+/* Non uniform medium: an exponential density profile along z, written as
+ * the atmospheres of the reference's example and test
+ * (examples/pumas/geometry.c:78-97, tests/test-pumas.c:1871-1884). */
+static double ref_locals_exponential(struct pumas_medium * medium,
+    struct pumas_state * state, struct pumas_locals * locals)
+{
+        const struct ref_medium * m = (const struct ref_medium *)medium;
+        locals->density =
+            m->density * exp(-(state->position[2] - m->z_ref) / m->scale);
+        const double uz = fabs(state->direction[2]);
+        return m->scale / ((uz <= m->cos_min) ? m->cos_min : uz);
+}
+
+/* The table entry of a device medium index when it has its own locals: the
+ * media of infinite and slab geometries and the air of a terrain. */
+static int ref_table_entry(const struct ref_geometry * r, int64_t k)
+{
+        if (r->g.kind == GEO_TERRAIN) return (k == 0) ? 0 : -1;
+        if (r->g.kind == GEO_VOXELS) return -1;
+        return (int)k;
 }
 
 /* The reference medium of device medium index k (its material, and a
@@ -163,6 +189,14 @@ static void ref_medium_fill(struct ref_geometry * r, int64_t k)
         geo_medium_properties(&r->g, (int)k, &material, &density);
         r->media[k].api.material = material;
         r->media[k].api.locals = (density > 0.) ? &ref_locals : NULL;
+        const int e = ref_table_entry(r, k);
+        if ((e >= 0) &&
+            (r->desc_media[e].locals == PUMAS_GPU_LOCALS_EXPONENTIAL)) {
+                r->media[k].scale = r->desc_media[e].scale;
+                r->media[k].z_ref = r->desc_media[e].z_ref;
+                r->media[k].cos_min = r->desc_media[e].cos_min;
+                r->media[k].api.locals = &ref_locals_exponential;
+        }
         __atomic_store(&r->media[k].density, &density, __ATOMIC_RELEASE);
 }
 
@@ -186,6 +220,10 @@ void * ref_geometry_create(const struct pumas_gpu_geometry_desc * d)
         g->n_media = d->n_media;
         r->material = calloc(d->n_media, sizeof(int));
         r->density = calloc(d->n_media, sizeof(double));
+        struct pumas_gpu_medium * table =
+            calloc(d->n_media, sizeof(struct pumas_gpu_medium));
+        memcpy(table, d->media, d->n_media * sizeof(struct pumas_gpu_medium));
+        r->desc_media = table;
         int i;
         for (i = 0; i < d->n_media; i++) {
                 r->material[i] = d->media[i].material;
@@ -237,6 +275,7 @@ void ref_geometry_destroy(void * geometry)
         free(r->media);
         free(r->material);
         free(r->density);
+        free((void *)r->desc_media);
         free(r);
 }
 

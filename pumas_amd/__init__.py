@@ -18,12 +18,12 @@ import os
 import numpy as np
 
 from . import _abi
-from ._abi import (Event, Flux, GeometryKind, HistoryStatus, Mode, Precision,
-                   Return, Step, STATE_DTYPE)
+from ._abi import (Event, Flux, GeometryKind, HistoryStatus, Locals, Mode,
+                   Precision, Return, Step, STATE_DTYPE)
 
 __all__ = [
     "Context", "Event", "Flux", "Geometry", "GeometryDescription",
-    "Recorder",
+    "Locals", "Medium", "Recorder",
     "GeometryKind", "HistoryStatus", "Mode", "Physics", "Precision", "PumasError", "Return",
     "STATE_DTYPE", "Server", "Step", "Tally", "TallySpec", "library",
     "new_states",
@@ -156,6 +156,44 @@ def _ptr(array, ctype):
     return array.ctypes.data_as(ctypes.POINTER(ctype))
 
 
+MEDIUM_DTYPE = np.dtype([("material", "i4"), ("density", "f8"),
+                         ("locals", "i4"), ("scale", "f8"), ("z_ref", "f8"),
+                         ("cos_min", "f8")], align=True)
+
+
+@dataclasses.dataclass(frozen=True)
+class Medium:
+    """A medium (struct pumas_gpu_medium): a material index and its local
+    properties, what the reference's `pumas_locals_cb` returns
+    (include/pumas.h:325-349). Uniform by default; `exponential` gives the
+    density profile density * exp(-(z - z_ref) / scale) with a proposed step
+    scale / max(|u_z|, cos_min), as the atmospheres of
+    examples/pumas/geometry.c:78-97 and tests/test-pumas.c:1871-1884."""
+    material: int
+    density: float = 0.0
+    locals: Locals = Locals.UNIFORM
+    scale: float = 0.0
+    z_ref: float = 0.0
+    cos_min: float = 0.0
+
+    @classmethod
+    def exponential(cls, material, density, scale, z_ref=0.0, cos_min=1.0):
+        return cls(material, density, Locals.EXPONENTIAL, scale, z_ref,
+                   cos_min)
+
+    def row(self):
+        return (self.material, self.density, int(self.locals), self.scale,
+                self.z_ref, self.cos_min)
+
+
+def _medium_row(m):
+    """A Medium, or a (material, density) pair for a uniform medium."""
+    if isinstance(m, Medium):
+        return m.row()
+    material, density = m
+    return (material, density, 0, 0.0, 0.0, 0.0)
+
+
 class GeometryDescription:
     """Host description of a device geometry (struct
     pumas_gpu_geometry_desc). The numpy arrays are kept alive here."""
@@ -165,10 +203,8 @@ class GeometryDescription:
                  z_top=0.0):
         self.kind = GeometryKind(kind)
         self.step_type = Step(step_type)
-        self.media = np.array(
-            [(m, d) for m, d in media],
-            dtype=np.dtype([("material", "i4"), ("density", "f8")],
-                           align=True))
+        self.media = np.array([_medium_row(m) for m in media],
+                              dtype=MEDIUM_DTYPE)
         self.z = None if z is None else np.ascontiguousarray(z, "f8")
         self.grid = grid or {}
         self.height = None if height is None else np.ascontiguousarray(

@@ -89,6 +89,12 @@ class GeometryKind(enum.IntEnum):
     VOXELS = 3
 
 
+class Locals(enum.IntEnum):
+    """enum pumas_gpu_locals."""
+    UNIFORM = 0
+    EXPONENTIAL = 1
+
+
 class Step(enum.IntEnum):
     """enum pumas_step, include/pumas.h:461-476."""
     CHECK = 0
@@ -126,7 +132,9 @@ class GpuContext(ctypes.Structure):
 
 
 class GpuMedium(ctypes.Structure):
-    _fields_ = [("material", ctypes.c_int), ("density", ctypes.c_double)]
+    _fields_ = [("material", ctypes.c_int), ("density", ctypes.c_double),
+                ("locals", ctypes.c_int), ("scale", ctypes.c_double),
+                ("z_ref", ctypes.c_double), ("cos_min", ctypes.c_double)]
 
 
 class GeometryDesc(ctypes.Structure):
@@ -261,6 +269,7 @@ EXPORTED_SYMBOLS = (
     "pumas_gpu_server_destroy",
     "pumas_gpu_server_submit",
     "pumas_gpu_server_poll",
+    "pumas_gpu_server_request_bytes",
     "pumas_gpu_server_wait",
     "pumas_gpu_error_message",
     "pumas_gpu_version",
@@ -336,6 +345,9 @@ def declare(lib):
     lib.pumas_gpu_server_submit.restype = ctypes.c_int
     lib.pumas_gpu_server_poll.argtypes = [vp, ctypes.c_uint64]
     lib.pumas_gpu_server_poll.restype = ctypes.c_int
+    lib.pumas_gpu_server_request_bytes.argtypes = [ctypes.c_size_t,
+                                                   ctypes.c_int]
+    lib.pumas_gpu_server_request_bytes.restype = ctypes.c_size_t
     lib.pumas_gpu_server_wait.argtypes = [vp, ctypes.c_uint64,
                                           P(RequestStatus)]
     lib.pumas_gpu_server_wait.restype = ctypes.c_int

@@ -53,7 +53,10 @@ struct Walker {
         int event, rc;
         int media0, end_medium;
         int medium, material, straight, step_index, step_max_type;
-        double density, step_max_medium;
+        /* density: the local density at the start of the next step;
+         * step_max_locals: the locals step times the accuracy (0: uniform
+         * medium). */
+        double density, step_max_medium, step_max_locals;
         double ti, wi, Xi, Xf, dei, grammage_max;
 };
 
@@ -270,14 +273,29 @@ struct EngineT {
                 return m;
         }
 
-        /* transport_set_locals, src/pumas.c:5528-5552: every device medium
-         * is uniform, hence a null locals step. */
-        PG_FN double locals_density(int medium, int * material) const
+        /* transport_set_locals, src/pumas.c:5528-5552: the local density
+         * of `medium` at the state's position, and the locals step times
+         * the context accuracy (zero for a uniform medium). The
+         * exponential profile is the device form of a non uniform
+         * `pumas_locals_cb`: density * exp(-(z - z_ref) / scale), proposing
+         * scale / max(|u_z|, cos_min). */
+        PG_FN double set_locals(const Track & t, int medium, int * material,
+            double * density) const
         {
-                double density;
-                geo_medium_properties(&g, medium, material, &density);
-                if (density <= 0.) density = p.material_density[*material];
-                return density;
+                double d;
+                geo_medium_properties(&g, medium, material, &d);
+                double step_max = 0.;
+                const int k = geo_locals_entry(&g, medium);
+                if ((k >= 0) && (g.locals[k].kind == GEO_LOCALS_EXPONENTIAL)) {
+                        const pg_locals & l = g.locals[k];
+                        d = d * lm::exp(-(t.pos[2] - l.z_ref) / l.scale);
+                        const double uz = fabs(t.dir[2]);
+                        step_max =
+                            l.scale / ((uz <= l.cos_min) ? l.cos_min : uz);
+                }
+                if (d <= 0.) d = p.material_density[*material];
+                *density = d;
+                return step_max * c.accuracy;
         }
 
         /* step_randn, src/pumas.c:7016-7029. */
@@ -659,17 +677,22 @@ struct EngineT {
                 *ratio = r;
         }
 
-        /* step_transport, src/pumas.c:6296-6925 (uniform media, no magnetic
-         * field). Returns the end medium through `out_medium`. */
+        /* step_transport, src/pumas.c:6296-6925 (no magnetic field).
+         * `density_io` holds the local density at the start of the step and
+         * receives the one at its end; `step_max_locals` the locals step,
+         * updated at the end of the step. Returns the end medium through
+         * `out_medium`. */
         PG_FN void step_transport(Track & t, int straight, int medium,
-            int material, double density, double grammage_max,
-            double step_max_medium, int step_max_type, int * out_medium) const
+            int material, double & density_io, double & step_max_locals,
+            double grammage_max, double step_max_medium, int step_max_type,
+            int * out_medium) const
         {
                 const int scheme = c.energy_loss;
                 double * const direction = t.dir;
                 double * const position = t.pos;
+                const double density = density_io;
                 const double density_i = 1. / density;
-                const double locals_density = density;
+                double locals_density = density;
                 const double momentum =
                     sqrt(t.energy * (t.energy + 2. * p.mass));
 
@@ -715,6 +738,8 @@ struct EngineT {
                 double step = (step_max_medium <= 0) ?
                     step_loc :
                     (step_max_medium < step_loc ? step_max_medium : step_loc);
+                if ((step_max_locals > 0.) && (step > step_max_locals))
+                        step = step_max_locals;
 
                 *out_medium = medium;
                 int event = EVENT_NONE;
@@ -790,6 +815,15 @@ struct EngineT {
                         }
                         event = EVENT_MEDIUM;
                         *out_medium = end_medium;
+                }
+
+                /* The end step locals, src/pumas.c:6494-6503: at the last
+                 * point inside the medium. */
+                if ((step_max_locals > 0.) &&
+                    ((step_max_type != STEP_RAW) || (event != EVENT_MEDIUM))) {
+                        int unused;
+                        step_max_locals =
+                            set_locals(t, medium, &unused, &locals_density);
                 }
 
                 if (event & EVENT_MEDIUM) {
@@ -1096,6 +1130,7 @@ struct EngineT {
                         } while ((mu > 1.) && !loop_stalled(t, tries));
                         rotate(t, mu);
                 }
+                density_io = locals_density;
         }
 
         /* Position of a material's first component in the global list. */
@@ -1764,8 +1799,12 @@ struct EngineT {
                     (step_max_type == STEP_CHECK))
                         step_max_medium += 0.5 * STEP_MIN;
                 int material;
-                const double density = locals_density(medium, &material);
-                const double step_max_locals = 0.;
+                double density;
+                const double step_max_locals =
+                    set_locals(t, medium, &material, &density);
+                if ((step_max_locals > 0.) &&
+                    (step_max_locals < step_max_medium))
+                        step_max_medium = step_max_locals;
                 if (density <= 0.) {
                         w.rc = RETURN_DENSITY_ERROR;
                         return;
@@ -1808,6 +1847,7 @@ struct EngineT {
                 w.material = material;
                 w.density = density;
                 w.step_max_medium = step_max_medium;
+                w.step_max_locals = step_max_locals;
                 w.step_max_type = step_max_type;
                 const int scheme = c.energy_loss;
                 w.straight = ((c.scattering == MODE_DISABLED) &&
@@ -1953,8 +1993,8 @@ struct EngineT {
 
                 int new_medium = -1;
                 step_transport(t, w.straight, w.medium, w.material, w.density,
-                    w.grammage_max, w.step_max_medium, w.step_max_type,
-                    &new_medium);
+                    w.step_max_locals, w.grammage_max, w.step_max_medium,
+                    w.step_max_type, &new_medium);
                 w.step_index++;
                 /* Recorded steps, src/pumas.c:5296-5328: every period-th
                  * step also gets the event-time weight update. */
@@ -1989,9 +2029,11 @@ struct EngineT {
                                 walker_finish(w);
                                 return;
                         }
-                        w.density = locals_density(w.medium, &w.material);
+                        w.step_max_locals =
+                            set_locals(t, w.medium, &w.material, &w.density);
                         w.straight = ((c.scattering == MODE_DISABLED) &&
-                                         (scheme <= MODE_MIXED)) ?
+                                         (scheme <= MODE_MIXED) &&
+                                         (w.step_max_locals <= 0.)) ?
                             1 :
                             0;
                         const int tmp_scheme =
@@ -2006,6 +2048,15 @@ struct EngineT {
                                 record_frame(w, w.medium, t.step_event);
                         walker_post(w);
                 }
+        }
+
+        /* The locals after a discrete vertex, src/pumas.c:5404-5415: the
+         * direction, and with it the proposed step, may have changed. */
+        PG_FN void walker_locals(Walker & w) const
+        {
+                if (w.step_max_locals > 0.)
+                        w.step_max_locals = set_locals(
+                            w.t, w.medium, &w.material, &w.density);
         }
 
         /* A pending inelastic DEL, src/pumas.c:5339-5386. */
@@ -2040,6 +2091,7 @@ struct EngineT {
                 if (rec_del) record_frame(w, w.medium, EVENT_NONE);
                 t.step_first = 1;
                 t.step_invlb1 = 0;
+                walker_locals(w);
                 walker_post(w);
         }
 
@@ -2055,6 +2107,7 @@ struct EngineT {
                         walker_finish(w);
                         return;
                 }
+                walker_locals(w);
                 walker_post(w);
         }
 

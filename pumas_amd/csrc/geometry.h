@@ -13,6 +13,7 @@
 #ifndef PUMAS_GEOMETRY_H
 #define PUMAS_GEOMETRY_H
 
+#include <math.h>
 #include <stdint.h>
 
 #if defined(__HIPCC__)
@@ -35,13 +36,23 @@ enum { GEO_STEP_CHECK = 0, GEO_STEP_RAW = 1 };
 /* Proposed step when no boundary lies ahead, in m. */
 #define GEO_STEP_FREE 1E+03
 
+/* Non uniform local properties of a media table entry
+ * (enum pumas_gpu_locals): kind 0 is uniform. */
+enum { GEO_LOCALS_UNIFORM = 0, GEO_LOCALS_EXPONENTIAL = 1 };
+struct pg_locals {
+        int kind;
+        double scale, z_ref, cos_min;
+};
+
 struct pg_geometry {
         int kind;
         int step_type;
-        /* Media table: material index and density (<= 0: material default). */
+        /* Media table: material index and density (<= 0: material default),
+         * and the local properties of each entry (NULL: all uniform). */
         int n_media;
         const int * material;
         const double * density;
+        const struct pg_locals * locals;
         /* Slabs. */
         int n_layers;
         const double * z;
@@ -73,6 +84,47 @@ GEO_FN double geo_terrain_height(
         const double h01 = h[g->nx], h11 = h[g->nx + 1];
         return (h00 * (1. - hx) + h10 * hx) * (1. - hy) +
             (h01 * (1. - hx) + h11 * hx) * hy;
+}
+
+/*
+ * Distance along pos + s u to the bilinear surface of cell (ix, iy): the
+ * smallest root s > 0 of z(s) - h(x(s), y(s)), a quadratic in s, or
+ * GEO_STEP_FREE when the line does not meet the surface patch's extension.
+ * In cell coordinates h = a + b hx + c hy + d hx hy (the bilinear form of
+ * geo_terrain_height), and hx, hy are linear in s. Roots past the cell walls
+ * are cut by the caller with the wall distances.
+ */
+GEO_FN double geo_terrain_crossing(const struct pg_geometry * g,
+    const double * pos, const double * u, int ix, int iy)
+{
+        const float * hp = g->height + (int64_t)iy * g->nx + ix;
+        const double h00 = hp[0], h10 = hp[1];
+        const double h01 = hp[g->nx], h11 = hp[g->nx + 1];
+        const double b = h10 - h00, c = h01 - h00;
+        const double d = (h11 - h10) - (h01 - h00);
+        const double hx = (pos[0] - (g->x0 + ix * g->dx)) / g->dx;
+        const double hy = (pos[1] - (g->y0 + iy * g->dy)) / g->dy;
+        const double ax = u[0] / g->dx, ay = u[1] / g->dy;
+        /* f(s) = A s^2 + B s + C. */
+        const double A = -d * ax * ay;
+        const double B = u[2] - (b + d * hy) * ax - (c + d * hx) * ay;
+        const double C = pos[2] - (h00 + b * hx + c * hy + d * hx * hy);
+        double r1 = -1., r2 = -1.;
+        if (A == 0.) {
+                if (B != 0.) r1 = -C / B;
+        } else {
+                /* The roots as q / A and C / q, without cancellation. */
+                const double disc = B * B - 4. * A * C;
+                if (disc < 0.) return GEO_STEP_FREE;
+                const double q = -0.5 * (B + ((B >= 0.) ? sqrt(disc) :
+                                                          -sqrt(disc)));
+                r1 = q / A;
+                if (q != 0.) r2 = C / q;
+        }
+        double s = GEO_STEP_FREE;
+        if ((r1 > 0.) && (r1 < s)) s = r1;
+        if ((r2 > 0.) && (r2 < s)) s = r2;
+        return s;
 }
 
 /*
@@ -131,18 +183,27 @@ GEO_FN int geo_locate(const struct pg_geometry * g, const double * pos,
                 const int cell = iy * (g->nx - 1) + ix;
                 *medium = (pos[2] < h) ? 1 + cell : 0;
                 if (step != 0) {
+                        /* The nearest of the cell walls, the top (air) or
+                         * bottom (rock) plane and the exact crossing of the
+                         * cell's bilinear surface. */
+                        const double u[3] = { sgn * dir[0], sgn * dir[1],
+                                sgn * dir[2] };
                         const double xl = g->x0 + ix * g->dx;
                         const double yl = g->y0 + iy * g->dy;
                         double s = geo_exit_distance(
-                            pos[0], sgn * dir[0], xl, xl + g->dx);
+                            pos[0], u[0], xl, xl + g->dx);
                         const double sy = geo_exit_distance(
-                            pos[1], sgn * dir[1], yl, yl + g->dy);
+                            pos[1], u[1], yl, yl + g->dy);
                         if (sy < s) s = sy;
-                        const double zhi = (*medium == 0) ? g->z_top : h;
-                        const double zlo = (*medium == 0) ? h : g->z0;
-                        const double sz =
-                            geo_exit_distance(pos[2], sgn * dir[2], zlo, zhi);
+                        double sz = GEO_STEP_FREE;
+                        if ((*medium == 0) && (u[2] > GEO_PARALLEL))
+                                sz = (g->z_top - pos[2]) / u[2];
+                        else if ((*medium > 0) && (u[2] < -GEO_PARALLEL))
+                                sz = (g->z0 - pos[2]) / u[2];
                         if (sz < s) s = sz;
+                        const double sh =
+                            geo_terrain_crossing(g, pos, u, ix, iy);
+                        if (sh < s) s = sh;
                         if (s < GEO_STEP_EPSILON) s = GEO_STEP_EPSILON;
                         *step = s;
                 }
@@ -183,6 +244,22 @@ GEO_FN int geo_locate(const struct pg_geometry * g, const double * pos,
         *medium = -1;
         if (step != 0) *step = -1.;
         return GEO_STEP_CHECK;
+}
+
+/* The media table entry whose local properties apply to a medium, or -1
+ * when the medium is uniform by construction (terrain ground columns and
+ * voxels, whose densities are per cell). */
+GEO_FN int geo_locals_entry(const struct pg_geometry * g, int medium)
+{
+        if (g->locals == 0) return -1;
+        switch (g->kind) {
+        case GEO_TERRAIN:
+                return (medium == 0) ? 0 : -1;
+        case GEO_VOXELS:
+                return -1;
+        default:
+                return medium;
+        }
 }
 
 /* Material index and density of a medium (density <= 0: default). */

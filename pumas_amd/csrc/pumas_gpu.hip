@@ -567,11 +567,35 @@ extern "C" enum pumas_return pumas_gpu_geometry_create(
                 return pg_fail(PUMAS_RETURN_VALUE_ERROR, "NULL argument");
         if ((desc->n_media <= 0) || (desc->media == NULL))
                 return pg_fail(PUMAS_RETURN_MEDIUM_ERROR, "no media");
+        bool uniform = true;
         for (int i = 0; i < desc->n_media; i++) {
-                const int m = desc->media[i].material;
+                const struct pumas_gpu_medium & medium = desc->media[i];
+                const int m = medium.material;
                 if ((m < 0) || (m >= physics->view.n_materials))
                         return pg_fail(PUMAS_RETURN_INDEX_ERROR,
                             "invalid material index [%d] for medium %d", m, i);
+                if (medium.locals == PUMAS_GPU_LOCALS_UNIFORM) continue;
+                if (medium.locals != PUMAS_GPU_LOCALS_EXPONENTIAL)
+                        return pg_fail(PUMAS_RETURN_VALUE_ERROR,
+                            "invalid locals kind %d for medium %d",
+                            (int)medium.locals, i);
+                if (!(medium.scale > 0.) || !(medium.scale < INFINITY) ||
+                    !(medium.cos_min > 0.) || !(medium.cos_min <= 1.) ||
+                    !(fabs(medium.z_ref) < INFINITY))
+                        return pg_fail(PUMAS_RETURN_VALUE_ERROR,
+                            "medium %d: an exponential profile needs a finite "
+                            "scale > 0, a finite z_ref and cos_min in (0, 1]",
+                            i);
+                const bool allowed =
+                    (desc->kind == PUMAS_GPU_GEOMETRY_INFINITE) ||
+                    (desc->kind == PUMAS_GPU_GEOMETRY_SLABS) ||
+                    ((desc->kind == PUMAS_GPU_GEOMETRY_TERRAIN) && (i == 0));
+                if (!allowed)
+                        return pg_fail(PUMAS_RETURN_VALUE_ERROR,
+                            "medium %d: non uniform locals apply to the media "
+                            "of infinite and slab geometries and to the air "
+                            "of a terrain", i);
+                uniform = false;
         }
 
         pumas_gpu_geometry * g = new pumas_gpu_geometry();
@@ -597,6 +621,17 @@ extern "C" enum pumas_return pumas_gpu_geometry_create(
         } while (0)
         UP(material.data(), material.size(), &v.material);
         UP(density.data(), density.size(), &v.density);
+        if (!uniform) {
+                std::vector<pg_locals> locals(desc->n_media);
+                for (int i = 0; i < desc->n_media; i++) {
+                        const struct pumas_gpu_medium & m = desc->media[i];
+                        locals[i].kind = (int)m.locals;
+                        locals[i].scale = m.scale;
+                        locals[i].z_ref = m.z_ref;
+                        locals[i].cos_min = m.cos_min;
+                }
+                UP(locals.data(), locals.size(), &v.locals);
+        }
 
         switch (desc->kind) {
         case PUMAS_GPU_GEOMETRY_INFINITE:
@@ -1560,9 +1595,10 @@ static enum pumas_return transport_impl(const pumas_gpu_physics * physics,
                     "lanes_per_wave must be at most 64 (got %d)",
                     lanes_per_wave);
         /* Every history takes the closed form CSDA when the geometry has
-         * no step limits (infinite medium), the energy loss is CSDA and
-         * scattering is disabled (src/pumas.c:3200-3230). */
+         * no step limits (a uniform infinite medium), the energy loss is
+         * CSDA and scattering is disabled (src/pumas.c:3200-3230). */
         const bool closed = (geometry->view.kind == GEO_INFINITE) &&
+            (geometry->view.locals == nullptr) &&
             (context->mode.energy_loss == PUMAS_MODE_CSDA) &&
             (context->mode.scattering == PUMAS_MODE_DISABLED);
         const Launch L = { n_cu, block, per_cu, rounds_per_check, graph,

@@ -37,11 +37,10 @@ using Clock = std::chrono::steady_clock;
 /* Device bytes of a request in flight: the scheduler's history record and
  * queue slots per history, plus the state and output copies of a
  * host-memory request. */
-size_t request_bytes(const struct pumas_gpu_request & r)
+size_t request_bytes(size_t n, int device)
 {
-        const size_t per =
-            pg_bytes_per_history() + (r.device ? 0 : 104 + 24);
-        return r.n * per + (size_t(1) << 20);
+        const size_t per = pg_bytes_per_history() + (device ? 0 : 104 + 24);
+        return n * per + (size_t(1) << 20);
 }
 
 struct Job {
@@ -283,7 +282,7 @@ extern "C" enum pumas_return pumas_gpu_server_submit(pumas_gpu_server * s,
                     "incomplete request (NULL physics, geometry or states)");
         auto * job = new Job();
         job->req = *request;
-        job->bytes = request_bytes(*request);
+        job->bytes = request_bytes(request->n, request->device);
         if (job->bytes > s->budget) {
                 const size_t b = job->bytes;
                 delete job;
@@ -308,6 +307,11 @@ extern "C" enum pumas_return pumas_gpu_server_submit(pumas_gpu_server * s,
         }
         s->cv_work.notify_all();
         return PUMAS_RETURN_SUCCESS;
+}
+
+extern "C" size_t pumas_gpu_server_request_bytes(size_t n, int device)
+{
+        return request_bytes(n, device);
 }
 
 extern "C" int pumas_gpu_server_poll(pumas_gpu_server * s, uint64_t ticket)

@@ -19,7 +19,8 @@ import math
 
 import numpy as np
 
-from . import Context, Event, GeometryDescription, Mode, STATE_DTYPE, Step
+from . import (Context, Event, GeometryDescription, GeometryKind, Medium,
+               Mode, STATE_DTYPE, Step)
 
 MUON_MASS = 0.10565839  # GeV, src/pumas.c:148
 
@@ -186,6 +187,46 @@ def slabs_backward_context(energy_loss=Mode.STRAGGLED):
 
 
 # ---------------------------------------------------------------------------
+# Non uniform media: standard rock under an exponential atmosphere, the
+# setting of examples/pumas/geometry.c (a flat rock layer and an atmosphere
+# whose locals callback returns a density falling with altitude and a
+# proposed step).
+# ---------------------------------------------------------------------------
+ATMOSPHERE_Z = (0.0, 200.0, 2000.0)
+
+
+def atmosphere_geometry(step_type=Step.RAW, scale=500.0, cos_min=0.05):
+    """Standard rock (2650 kg/m^3) in z in [0, 200) m under air in
+    [200, 2000) m with density 1.205 exp(-(z - 200) / scale) kg/m^3 and the
+    proposed step scale / max(|u_z|, cos_min)."""
+    air = Medium.exponential(AIR, 1.205, scale, z_ref=200.0, cos_min=cos_min)
+    return GeometryDescription.slabs(ATMOSPHERE_Z, [(ROCK, 2650.0), air],
+                                     step_type=step_type)
+
+
+def atmosphere_states(n, seed=0, first=0, kmin=1.0, kmax=1e3, cmin=0.3,
+                      cmax=1.0, forward=True):
+    """Forward: initial states at the top of the air (z = 1999 m) going
+    down; backward: final states at the bottom of the rock (z = 1 mm) going
+    down, weight 1/pdf. K log-uniform in [kmin, kmax] GeV, cos(zenith)
+    uniform in [cmin, cmax], charge -1."""
+    s = (slabs_forward_states if forward else slabs_backward_states)(
+        n, seed=seed, first=first, kmin=kmin, kmax=kmax, cmin=cmin,
+        cmax=cmax)
+    if forward:
+        s["position"][:, 2] = 1999.0
+    return s
+
+
+def exponential_infinite_geometry(scale=100.0, cos_min=1.0):
+    """One infinite water medium of density 1000 exp(-z / scale) kg/m^3
+    (no boundary: every step is set by the physics and the locals)."""
+    return GeometryDescription(
+        GeometryKind.INFINITE,
+        [Medium.exponential(WATER, 1000.0, scale, cos_min=cos_min)])
+
+
+# ---------------------------------------------------------------------------
 # Terrain: heightmap of seeded fractal value noise, backward muography.
 # ---------------------------------------------------------------------------
 def value_noise_2d(nx, ny, seed, octaves=6, base_cells=4, gain=0.5):
@@ -223,10 +264,11 @@ TERRAIN_DETECTOR_Z = -100.0
 
 
 def terrain_geometry(n=1024, spacing=5.0, relief=800.0, seed=0,
-                     rho_min=2000.0, rho_max=2900.0):
+                     rho_min=2000.0, rho_max=2900.0, air=(AIR, 0.0)):
     """Heightmap of n x n nodes at `spacing` m with heights in [0, relief] m
     and one standard rock column per cell with a density uniform in
-    [rho_min, rho_max] kg/m^3; air above, to TERRAIN_ZTOP."""
+    [rho_min, rho_max] kg/m^3; air above, to TERRAIN_ZTOP (`air`: a Medium,
+    or a (material, density) pair)."""
     h = relief * value_noise_2d(n, n, seed)
     ids = np.arange((n - 1) ** 2, dtype=np.uint64)
     rho = rho_min + (rho_max - rho_min) * uniform(seed, 3, ids, 0)
@@ -234,7 +276,7 @@ def terrain_geometry(n=1024, spacing=5.0, relief=800.0, seed=0,
     return GeometryDescription.terrain(
         h.astype(np.float32), rho.reshape(n - 1, n - 1).astype(np.float32),
         x0=-half, y0=-half, dx=spacing, dy=spacing, z0=TERRAIN_Z0,
-        z_top=TERRAIN_ZTOP, air=(AIR, 0.0), ground_material=ROCK)
+        z_top=TERRAIN_ZTOP, air=air, ground_material=ROCK)
 
 
 def terrain_states(n, extent, seed=0, first=0, kmin=1e-2, kmax=1e2,

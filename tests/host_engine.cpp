@@ -88,6 +88,13 @@ extern "C" int host_transport(const pumas_gpu_context *ctx, const pumas_gpu_geom
     std::vector<int> mat(d->n_media); std::vector<double> den(d->n_media);
     for (int i = 0; i < d->n_media; i++) { mat[i] = d->media[i].material; den[i] = d->media[i].density; }
     g.kind = d->kind; g.step_type = d->step_type; g.n_media = d->n_media; g.material = mat.data(); g.density = den.data();
+    std::vector<pg_locals> loc(d->n_media); bool uniform = true;
+    for (int i = 0; i < d->n_media; i++) {
+        loc[i].kind = d->media[i].locals; loc[i].scale = d->media[i].scale;
+        loc[i].z_ref = d->media[i].z_ref; loc[i].cos_min = d->media[i].cos_min;
+        if (loc[i].kind != GEO_LOCALS_UNIFORM) uniform = false;
+    }
+    g.locals = uniform ? nullptr : loc.data();
     g.n_layers = d->n_layers; g.z = d->z; g.nx = d->nx; g.ny = d->ny; g.nz = d->nz; g.x0 = d->x0; g.y0 = d->y0;
     g.z0 = d->z0; g.dx = d->dx; g.dy = d->dy; g.dz = d->dz; g.height = d->height; g.cell_density = d->density;
     g.label = d->label; g.z_top = d->z_top;

@@ -15,7 +15,8 @@ Media agree at 10^6 random points away from the surfaces (|z - h| > 1e-6
 m, 1e-9 m from walls). Raw steps (slabs, voxels) are checked by marching
 along the ray: the medium is the start one up to 1e-7 m before the proposed
 step and another one at it. Checked steps (terrain) never cross a cell
-wall before they end.
+wall before they end, and end on the bilinear surface when they cross it
+(brute force root search along 10^5 rays).
 """
 import ctypes
 import os
@@ -139,7 +140,7 @@ def test_terrain_full_grid(geo, terrain):
     np.testing.assert_array_equal(m[clear], want[clear])
     assert (m == 0).any() and (m > 0).any() and (m < 0).any()
     # A checked step stays in its cell of the grid (the transport checks
-    # the medium at its end and bisects on a surface crossing).
+    # the medium at its end).
     inside = (m >= 0) & clear
     ix0, iy0 = np.floor(fx), np.floor(fy)
     for frac in np.linspace(0.0, 1.0, 17):
@@ -148,6 +149,84 @@ def test_terrain_full_grid(geo, terrain):
         jy = np.floor((p[:, 1] - g["y0"]) / g["dy"])
         ok = (jx == ix0) & (jy == iy0)
         assert ok[inside].all()
+
+
+def test_terrain_surface_crossings(geo, terrain):
+    """The proposed step ends on the bilinear surface when it crosses it
+    in the cell: brute force search of the first sign change of z - h along
+    each ray (129 samples up to the cell wall or the top / bottom plane,
+    then bisection on scipy's interpolated surface). No step passes a
+    crossing, and a step shorter than the wall and plane distances ends on
+    the surface."""
+    from scipy.interpolate import RegularGridInterpolator
+    rng = np.random.default_rng(4)
+    g = terrain.grid
+    n = 100_000
+    half = 0.5 * (g["nx"] - 1) * g["dx"]
+    xs = g["x0"] + g["dx"] * np.arange(g["nx"])
+    ys = g["y0"] + g["dy"] * np.arange(g["ny"])
+    surface = RegularGridInterpolator(
+        (ys, xs), terrain.height.astype(np.float64), method="linear")
+    # Points within 3 m of the surface, going downwards from the air and
+    # upwards from the rock, so that many rays cross it in their cell.
+    pos = np.stack([rng.uniform(-0.9 * half, 0.9 * half, n),
+                    rng.uniform(-0.9 * half, 0.9 * half, n),
+                    np.zeros(n)], 1)
+    h0 = surface(pos[:, 1::-1])
+    pos[:, 2] = h0 + rng.uniform(-3.0, 3.0, n)
+    dirs = random_dirs(rng, n)
+    dirs[:, 2] = -np.sign(pos[:, 2] - h0) * np.abs(dirs[:, 2])
+    m, s, st = locate(geo, terrain, pos, dirs)
+    assert (st == 0).all() and (m >= 0).all()
+
+    def f(t):
+        p = pos + t[:, None] * dirs
+        return p[:, 2] - surface(p[:, 1::-1])
+
+    # Distance to the cell walls and to the top (air) or bottom (rock).
+    fx = (pos[:, 0] - g["x0"]) / g["dx"]
+    fy = (pos[:, 1] - g["y0"]) / g["dy"]
+    lim = np.full(n, np.inf)
+    for k, fk, d0, dk in ((0, fx, g["x0"], g["dx"]), (1, fy, g["y0"], g["dy"])):
+        lo = d0 + np.floor(fk) * dk
+        with np.errstate(divide="ignore"):
+            t = np.where(dirs[:, k] > 0, (lo + dk - pos[:, k]) / dirs[:, k],
+                         (lo - pos[:, k]) / dirs[:, k])
+        lim = np.minimum(lim, np.where(np.abs(dirs[:, k]) > 1e-12, t, np.inf))
+    plane = np.where(m == 0, (terrain.z_top - pos[:, 2]) / dirs[:, 2],
+                     (g["z0"] - pos[:, 2]) / dirs[:, 2])
+    lim = np.minimum(lim, np.where(plane > 0, plane, np.inf))
+    f0 = f(np.zeros(n))
+    side = np.sign(f0)
+    ok = np.abs(f0) > 1e-6
+    # First sign change on a grid of the segment [0, lim].
+    ts = np.linspace(0.0, 1.0, 129)[:, None] * lim[None, :]
+    fs = np.stack([f(t) for t in ts])
+    changed = np.sign(fs) != side[None, :]
+    has = changed.any(axis=0)
+    k = np.argmax(changed, axis=0)
+    a = np.where(has, ts[np.maximum(k - 1, 0), np.arange(n)], 0.0)
+    b = np.where(has, ts[k, np.arange(n)], 0.0)
+    for _ in range(60):
+        c = 0.5 * (a + b)
+        same = np.sign(f(c)) == side
+        a = np.where(same, c, a)
+        b = np.where(same, b, c)
+    s_brute = np.where(has, 0.5 * (a + b), np.inf)
+    cross = ok & has
+    assert cross.mean() > 0.3
+    print(f"{int(cross.sum())} of {n} rays cross the surface in their cell")
+    # No proposed step passes a crossing.
+    assert (s[ok] <= s_brute[ok] + 1e-7).all()
+    # Where the crossing comes before the walls and planes, the step ends on
+    # it (a tangent touch without a sign change aside: f(s) ~ 0 there too).
+    by_surface = ok & (s < lim - 1e-6)
+    fe = f(np.where(by_surface, s, 0.0))
+    assert (np.abs(fe[by_surface]) < 1e-7).all()
+    np.testing.assert_allclose(s[cross & by_surface],
+                               s_brute[cross & by_surface], rtol=0,
+                               atol=1e-6)
+    assert (cross & ~by_surface & (s_brute < lim - 1e-6)).sum() == 0
 
 
 @pytest.fixture(scope="module")

@@ -60,7 +60,8 @@ def test_budget_admission(gpu_standard, geometries):
     """A budget that holds one request at a time still completes every
     request, in order of submission, with identical results."""
     reqs = mix(seed=1, duration=0.04, nmax=5e3)
-    budget = max(r.n for r in reqs) * 476 + (1 << 20) + 1
+    budget = pa.library().pumas_gpu_server_request_bytes(
+        max(r.n for r in reqs), 0) + 1
     with pa.Server(hbm_budget=budget, slots=4) as srv:
         tickets = []
         for r in reqs:

@@ -21,6 +21,7 @@ import pumas_amd as pa
 from pumas_amd import _abi
 from pumas_amd import workloads as W
 
+import locals_cases
 import make_golden
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -106,6 +107,38 @@ def test_golden_cases_bit_exact(host, ref_standard, case):
                         orc.RefGeometry(desc), states, seed=0,
                         threads=8)
     check_exact(out, ref)
+
+
+@pytest.mark.parametrize("case", [c[0] for c in locals_cases.cases()])
+def test_non_uniform_locals_bit_exact(host, ref_standard, case):
+    """Non uniform media (exponential density profiles with a locals
+    step): the engine against the reference with the equivalent
+    `pumas_locals_cb`, every history bit identical."""
+    import oracle as orc
+    name, pname, ctx, desc, states = next(
+        c for c in locals_cases.cases() if c[0] == case)
+    out = run_host(host, pname, ctx, desc, states)
+    ref = orc.transport(ref_standard, ctx, orc.RefGeometry(desc), states,
+                        seed=0, threads=8)
+    print(f"{case}: {int(ref.steps.sum())} geometry steps, events "
+          f"{np.unique(ref.events)}")
+    check_exact(out, ref)
+
+
+def test_non_uniform_locals_change_the_transport(ref_standard):
+    """The locals cases exercise the non uniform path: the reference's
+    results change when the exponential media are made uniform."""
+    import oracle as orc
+    for name, pname, ctx, desc, states in locals_cases.cases(128):
+        m = desc.media
+        uniform = pa.GeometryDescription(
+            desc.kind, [(int(a), float(b)) for a, b in
+                        zip(m["material"], m["density"])],
+            step_type=desc.step_type, z=desc.z)
+        a = orc.transport(ref_standard, ctx, orc.RefGeometry(desc), states)
+        b = orc.transport(ref_standard, ctx, orc.RefGeometry(uniform), states)
+        assert (a.states["position"] != b.states["position"]).any(), name
+        assert a.steps.sum() > b.steps.sum(), name
 
 
 def test_terrain_bit_exact(host, ref_standard):
