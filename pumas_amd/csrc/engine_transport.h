@@ -329,39 +329,26 @@ struct EngineT {
                 if (stsq <= 0.) return;
                 const double st = sqrt(stsq);
 
-                double u0x = 0., u0y = 0., u0z = 0.;
-                const double a0 = fabs(direction[0]);
-                const double a1 = fabs(direction[1]);
-                const double a2 = fabs(direction[2]);
-                if (a0 > a1) {
-                        if (a0 > a2) {
-                                const double nrm =
-                                    1. / sqrt(direction[0] * direction[0] +
-                                             direction[2] * direction[2]);
-                                u0x = -direction[2] * nrm,
-                                u0z = direction[0] * nrm;
-                        } else {
-                                const double nrm =
-                                    1. / sqrt(direction[1] * direction[1] +
-                                             direction[2] * direction[2]);
-                                u0y = direction[2] * nrm,
-                                u0z = -direction[1] * nrm;
-                        }
-                } else {
-                        if (a1 > a2) {
-                                const double nrm =
-                                    1. / sqrt(direction[0] * direction[0] +
-                                             direction[1] * direction[1]);
-                                u0x = direction[1] * nrm,
-                                u0y = -direction[0] * nrm;
-                        } else {
-                                const double nrm =
-                                    1. / sqrt(direction[1] * direction[1] +
-                                             direction[2] * direction[2]);
-                                u0y = direction[2] * nrm,
-                                u0z = -direction[1] * nrm;
-                        }
-                }
+                /* The reference's choice of the axis pair (the two
+                 * components other than the smallest), with the pair
+                 * selected first and normalised once: lanes of a wavefront
+                 * with different pairs then run the same instructions. The
+                 * operations of each pair are unchanged. */
+                const double d0 = direction[0], d1 = direction[1],
+                             d2 = direction[2];
+                const double a0 = fabs(d0), a1 = fabs(d1), a2 = fabs(d2);
+                const bool pair_xz = (a0 > a1) && (a0 > a2);
+                const bool pair_xy = !(a0 > a1) && (a1 > a2);
+                const bool pair_yz = !pair_xz && !pair_xy;
+                const double e0 = pair_yz ? d1 : d0;
+                const double e1 = pair_xy ? d1 : d2;
+                const double nrm = 1. / sqrt(e0 * e0 + e1 * e1);
+                const double u0x =
+                    pair_xz ? -d2 * nrm : (pair_xy ? d1 * nrm : 0.);
+                const double u0y =
+                    pair_yz ? d2 * nrm : (pair_xy ? -d0 * nrm : 0.);
+                const double u0z =
+                    pair_xz ? d0 * nrm : (pair_yz ? -d1 * nrm : 0.);
                 const double u1x = u0y * direction[2] - u0z * direction[1];
                 const double u1y = u0z * direction[0] - u0x * direction[2];
                 const double u1z = u0x * direction[1] - u0y * direction[0];
