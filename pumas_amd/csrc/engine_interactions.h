@@ -316,6 +316,21 @@ struct CoulombData {
         double fCM[2];
 };
 
+/*
+ * The element screening data hold at most 3 atomic terms and the nuclear
+ * one (screening[n_parameters - 1]). Loops over the terms run to the fixed
+ * bound with a guard, and the nuclear term is selected by comparisons, so
+ * that the arrays are indexed by constants and stay in registers; the
+ * operations of each term are unchanged.
+ */
+PG_FN double screening_at(const CoulombData & cd, int k)
+{
+        return (k == 0) ? cd.screening[0] :
+            (k == 1)    ? cd.screening[1] :
+            (k == 2)    ? cd.screening[2] :
+                          cd.screening[3];
+}
+
 /* coulomb_frame_parameters, src/pumas.c:7492-7510. */
 PG_FN void coulomb_frame_parameters(double Z, double A, double mass,
     double kinetic, double * kinetic0, double * parameters)
@@ -371,13 +386,18 @@ PG_FN void coulomb_screening_parameters(const PhysicsView & p, int iel,
         const double p02 = kinetic0 * (kinetic0 + 2. * mass);
         const double d = 0.25 * HBAR_C * HBAR_C / p02 * ktv;
         const int n = (int)p.element_nscr[iel];
-        for (int i = 0; i < n; i++) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+                if (i >= n) break;
                 amplitude[i] = p.element_amplitude[3 * iel + i];
                 const double a = p.element_exponent[3 * iel + i] / BOHR_RADIUS;
                 screening[i] = d * a * a;
         }
         const double RN = p.element_rn[iel];
-        screening[n] = 12. * d / (RN * RN);
+        const double nuclear = 12. * d / (RN * RN);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+                if (i == n) screening[i] = nuclear;
         *n_parameters = n + 1;
 }
 
@@ -418,11 +438,15 @@ PG_FN void coulomb_pole_reduction(CoulombData & cd)
         for (int i = 0; i < 3; i++) cd.a[i] = cd.b[i] = 0.;
         for (int i = 0; i < 4; i++) cd.c[i] = 0.;
 
-        for (int i = 0; i < n; i++) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+                if (i >= n) break;
                 const double Ai = cd.amplitude[i];
                 const double Bi = cd.screening[i];
                 cd.b[i] = Ai * Ai;
-                for (int j = i + 1; j < n; j++) {
+#pragma unroll
+                for (int j = i + 1; j < 3; j++) {
+                        if (j >= n) break;
                         const double Aj = cd.amplitude[j];
                         const double Bj = cd.screening[j];
                         const double d = 2 * Ai * Aj / (Bj - Bi);
@@ -431,9 +455,11 @@ PG_FN void coulomb_pole_reduction(CoulombData & cd)
                 }
         }
 
-        const double N = cd.screening[n];
+        const double N = screening_at(cd, n);
         double Sa[4] = { 0, 0, 0, 0 }, Sb[4] = { 0, 0, 0, 0 };
-        for (int i = 0; i < n; i++) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+                if (i >= n) break;
                 const double Bi = cd.screening[i];
                 const double x = 1. - Bi / N;
                 double r = x;
@@ -450,7 +476,9 @@ PG_FN void coulomb_pole_reduction(CoulombData & cd)
                 tmp *= N;
         }
 
-        for (int i = 0; i < n; i++) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+                if (i >= n) break;
                 const double Bi = cd.screening[i];
                 double d = 1. - Bi / N;
                 d *= d;
@@ -467,7 +495,9 @@ PG_FN double coulomb_cs0(double mu, const CoulombData & cd)
         const double S = cd.fspin;
         const int n = cd.n_parameters - 1;
         double cs0 = 0.;
-        for (int i = 0; i < n; i++) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+                if (i >= n) break;
                 const double alp = cd.screening[i];
                 const double r = mu / (mu + alp);
                 const double L = lm::log(1. + mu / alp);
@@ -478,7 +508,7 @@ PG_FN double coulomb_cs0(double mu, const CoulombData & cd)
                 cs0 += cd.a[i] * (J0 - S * J1) + cd.b[i] * (I0 - S * I1);
         }
 
-        const double N = cd.screening[n];
+        const double N = screening_at(cd, n);
         const double r = mu / (mu + N);
         const double L = lm::log(1. + mu / N);
         const double I0 = r / N;
@@ -501,7 +531,7 @@ PG_FN double coulomb_cs0(double mu, const CoulombData & cd)
 /* coulomb_restricted_cs, src/pumas.c:7591-7610. */
 PG_FN double coulomb_restricted_cs(double mu, const CoulombData & cd)
 {
-        if ((mu >= 1.) || (mu >= 1E+06 * cd.screening[cd.n_parameters - 1]))
+        if ((mu >= 1.) || (mu >= 1E+06 * screening_at(cd, cd.n_parameters - 1)))
                 return 0.;
         const double cs0 = coulomb_cs0(1., cd);
         const double cs1 = coulomb_cs0(mu, cd);

@@ -1568,7 +1568,7 @@ struct EngineT {
                 const int n = d.n_parameters - 1;
                 const float Af = (float)A, mu0f = (float)mu0;
                 const float fspin = (float)d.fspin;
-                const float N = (float)d.screening[n];
+                const float N = (float)screening_at(d, n);
                 float amp[3], scr[3];
                 for (int i = 0; i < 3; i++) {
                         amp[i] = (i < n) ? (float)d.amplitude[i] : 0.f;
@@ -1672,7 +1672,9 @@ struct EngineT {
                 coulomb_parameters(d, material, ihard, kinetic, &kinetic0);
                 const int n = d.n_parameters - 1;
                 double A = d.screening[0];
-                for (int i = 1; i < n; i++) {
+#pragma unroll
+                for (int i = 1; i < 3; i++) {
+                        if (i >= n) break;
                         const double Ai = d.screening[i];
                         if (Ai < A) A = Ai;
                 }
@@ -1696,12 +1698,14 @@ struct EngineT {
                                 mu1 = 1;
 
                         double ratio = 0.;
-                        for (int i = 0; i < n; i++) {
+#pragma unroll
+                        for (int i = 0; i < 3; i++) {
+                                if (i >= n) break;
                                 ratio += d.amplitude[i] * (A + mu1) /
                                     (d.screening[i] + mu1);
                         }
                         ratio *= coulomb_nuclear_form_factor(
-                            mu1, d.screening[n]);
+                            mu1, screening_at(d, n));
                         ratio *= ratio * (1. - d.fspin * mu1);
                         if (t.rng.uniform() <= ratio) break;
                         if (loop_stalled(t, tries)) break;
