@@ -510,12 +510,15 @@ def main():
             args.precision == "fp64"):
         with open(tpath) as f:
             prof = json.load(f)
-        if prof.get("launch") == tuned:
-            traffic = prof.get("bytes_per_launch")
+        # One entry per profiled launch, or a single profile.
+        entries = prof.get("profiles", [prof])
+        match = [e for e in entries if e.get("launch") == tuned]
+        if match:
+            traffic = match[0].get("bytes_per_launch")
             traffic_note = prof.get("note")
         else:
-            traffic_note = (f"profile launch {prof.get('launch')} differs "
-                            f"from the timed one {tuned}")
+            traffic_note = (f"profiled launches {[e.get('launch') for e in entries]}"
+                            f" differ from the timed one {tuned}")
 
     serve = None
     if args.serve_seconds > 0 and args.workload == "slabs":
