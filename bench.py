@@ -579,6 +579,15 @@ def main():
             "measured_bytes_per_unit": (
                 traffic * launches / (n * args.steps if csda else
                                       total_steps) if traffic else None),
+            "bound_note": (
+                "frac is the algorithmic-bytes roofline; the kernels are not "
+                "bandwidth bound: the stepping kernel keeps a history in "
+                "registers between DEL vertices (a few B of HBM traffic per "
+                "step measured) and is bound by VALU issue over divergent "
+                "lanes (VALUBusy ~66 %, VALUUtilization ~35 %); the closed "
+                "form entry kernel moves exactly its 208 B per history and "
+                "is latency bound (VALUBusy ~25 %); profiles/ holds the "
+                "counters"),
         },
         "transport_ms_per_pass": mean_ms,
         "history_status": {

@@ -109,6 +109,11 @@ enum pumas_return run_host_request(const struct pumas_gpu_request & r,
         q += ns * sizeof(uint32_t);
         if (o.rc) d_out.rc = reinterpret_cast<int32_t *>(q);
 
+        /* Outputs start at zero: a transport that fails before its first
+         * kernel (a context error) leaves them so, and they are copied back
+         * all the same. */
+        HIP_TRY_FREE(hipMemsetAsync(p + ((b_states + 255) & ~size_t(255)), 0,
+            b_out, stream));
         HIP_TRY_FREE(hipMemcpyAsync(d_states, r.states, n * sizeof(*r.states),
             hipMemcpyHostToDevice, stream));
         const enum pumas_return rc = pumas_gpu_transport(r.physics,
