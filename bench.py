@@ -22,8 +22,9 @@ the host cores, over a bounded sample of the same workload.
 
 Other workloads (`--workload`, not the default line): `csda` — backward
 CSDA through 1 km of standard rock, 10^8 histories (closed form, one
-muon-step per history); `terrain` — backward straggled muography along 64
-lines of sight (10^3..10^6 events each) under a 1024 x 1024 heightmap;
+muon-step per history); `terrain` — backward straggled muography along
+4096 lines of sight (`--lines`, 10^3..10^6 events each) under a 1024 x 1024
+heightmap;
 `voxels` — backward straggled transport through 512^3 voxels of 2 m at a 1%
 cutoff, 10^8 histories (10^7 generated states tiled on the device, each
 history with its own id and random stream). `--precision fp32` selects
@@ -325,9 +326,14 @@ def setup(args, pa, W):
                 lines, n)
     n = args.n or 100_000_000
     m = min(n, 10_000_000)
-    log(f"bench: generating the 512^3 voxel grid and {m} states")
-    return ("iron_c001", W.voxel_geometry_large(), W.voxel_context(),
-            W.voxel_states(m, 512 * 2.0), n)
+    log("bench: generating the 512^3 voxel grid")
+    t0 = time.perf_counter()
+    desc = W.voxel_geometry_large()
+    log(f"bench: voxel grid ready ({time.perf_counter() - t0:.0f} s), "
+        f"generating {m} states")
+    states = W.voxel_states(m, 512 * 2.0)
+    log(f"bench: states ready ({time.perf_counter() - t0:.0f} s)")
+    return ("iron_c001", desc, W.voxel_context(), states, n)
 
 
 def main():

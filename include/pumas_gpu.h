@@ -320,6 +320,14 @@ struct pumas_gpu_options {
          * requests); 64: full wavefronts on as few CUs as the batch fills.
          * Results do not depend on it. */
         int lanes_per_wave;
+        /* Iterations after which a rejection or bisection loop gives up
+         * and its history ends with PUMAS_GPU_HISTORY_LOOP_LIMIT (0 =
+         * 2^16). The reference's loops have no limit: above its tables
+         * (kinetic energies beyond 1e9 GeV, which backward discrete losses
+         * can reach) its elastic hard scattering angle loop needs up to
+         * ~3e7 trials per event, a wavefront held for seconds. Parity on
+         * such histories needs 2^25. */
+        int max_loop;
 };
 
 /*
@@ -349,9 +357,11 @@ enum pumas_gpu_history_status {
         /* options->max_steps Monte Carlo steps were taken (the reference has
          * no cap). */
         PUMAS_GPU_HISTORY_STEP_LIMIT = 100,
-        /* A rejection or bisection loop reached 2^25 iterations (a non
-         * finite input, or kinetic energies far above the tables, where
-         * the reference would not return or needs more trials). */
+        /* A rejection or bisection loop reached options->max_loop
+         * iterations (2^16 by default): a non finite input, a sampling
+         * range that rounds to nothing, or kinetic energies far above the
+         * tables, where the reference would not return or needs more
+         * trials. */
         PUMAS_GPU_HISTORY_LOOP_LIMIT = 101,
         /* The target selection of a discrete energy loss matched no target
          * (the reference aborts on assert(0), src/pumas.c:6207, 6264). */

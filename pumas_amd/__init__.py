@@ -339,7 +339,7 @@ class TransportStats:
 
 def _options(precision, block_size, blocks_per_cu, max_steps,
              rounds_per_check=0, eager=False, ehs_batch=0, recorder=None,
-             tail_del=0, lds_tables=0, lanes_per_wave=0):
+             tail_del=0, lds_tables=0, lanes_per_wave=0, max_loop=0):
     o = _abi.GpuOptions()
     o.precision = int(precision)
     o.block_size = block_size
@@ -352,6 +352,7 @@ def _options(precision, block_size, blocks_per_cu, max_steps,
     o.tail_del = int(tail_del)
     o.lds_tables = int(lds_tables)
     o.lanes_per_wave = int(lanes_per_wave)
+    o.max_loop = int(max_loop)
     return o
 
 
@@ -359,7 +360,7 @@ def transport(physics, context, geometry, states, seed=0, first_history=0,
               outputs=True, precision=Precision.FP64, block_size=0,
               blocks_per_cu=0, max_steps=0, raise_errors=True,
               rounds_per_check=0, eager=False, ehs_batch=0, recorder=None,
-              tail_del=0, lds_tables=0, lanes_per_wave=0):
+              tail_del=0, lds_tables=0, lanes_per_wave=0, max_loop=0):
     """Transport host `states` (a STATE_DTYPE array, updated in place).
 
     Batch counterpart of pumas_context_transport: history i draws from the
@@ -388,7 +389,7 @@ def transport(physics, context, geometry, states, seed=0, first_history=0,
     stats = _abi.GpuStats()
     opts = _options(precision, block_size, blocks_per_cu, max_steps,
                     rounds_per_check, eager, ehs_batch, recorder, tail_del,
-                    lds_tables, lanes_per_wave)
+                    lds_tables, lanes_per_wave, max_loop)
     ctx = context.to_c()
     rc = lib.pumas_gpu_transport_host(
         physics.handle, ctypes.byref(ctx), geometry.handle,
@@ -411,7 +412,7 @@ def transport_device(physics, context, geometry, states_ptr, n, seed=0,
                      block_size=0, blocks_per_cu=0, max_steps=0, stream=None,
                      stats=True, rounds_per_check=0, eager=False,
                      ehs_batch=0, recorder=None, tail_del=0,
-                     lds_tables=0, lanes_per_wave=0):
+                     lds_tables=0, lanes_per_wave=0, max_loop=0):
     """Transport `n` states already in device memory at `states_ptr`.
 
     `outputs` maps any of "events", "media", "steps", "draws", "rc" to a
@@ -425,7 +426,7 @@ def transport_device(physics, context, geometry, states_ptr, n, seed=0,
     st = _abi.GpuStats()
     opts = _options(precision, block_size, blocks_per_cu, max_steps,
                     rounds_per_check, eager, ehs_batch, recorder, tail_del,
-                    lds_tables, lanes_per_wave)
+                    lds_tables, lanes_per_wave, max_loop)
     ctx = context.to_c()
     _check(lib.pumas_gpu_transport(
         physics.handle, ctypes.byref(ctx), geometry.handle,

@@ -165,7 +165,7 @@ class GpuOptions(ctypes.Structure):
                 ("rounds_per_check", ctypes.c_int), ("eager", ctypes.c_int),
                 ("ehs_batch", ctypes.c_int), ("recorder", ctypes.c_void_p),
                 ("tail_del", ctypes.c_int), ("lds_tables", ctypes.c_int),
-                ("lanes_per_wave", ctypes.c_int)]
+                ("lanes_per_wave", ctypes.c_int), ("max_loop", ctypes.c_int)]
 
 
 class GpuStats(ctypes.Structure):

@@ -229,6 +229,8 @@ struct Config {
         int event;
         double limit_energy, limit_distance, limit_grammage, limit_time;
         double accuracy;
+        /* Iterations after which a rejection or bisection loop gives up. */
+        int loop_max;
 };
 
 /* Table accessors, src/pumas.c:4260-4752. */

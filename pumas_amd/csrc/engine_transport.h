@@ -91,28 +91,29 @@ struct Recorder {
  * The rejection and bisection loops of the reference have no iteration
  * limit. For finite inputs within the tables each of them accepts with a
  * probability bounded away from zero (at most 120 iterations over 10^7
- * voxel histories). Above the tables (kinetic energies beyond 1e9 GeV) the
+ * voxel histories). Above the tables (kinetic energies beyond 1e9 GeV,
+ * which backward discrete losses can reach from a few hundred GeV) the
  * elastic hard scattering angle loop accepts ever more rarely (the nuclear
  * form factor cuts the large angles): the reference needs up to 3e7 trials
- * around 1e10 GeV, and at 1e10 GeV forward one of its loops never ends; a
- * NaN or infinite input can also make a loop never accept. A single such
- * lane would keep its wavefront, and so the launch, from ending. After
- * LOOP_MAX iterations a loop gives up: its last values are used once and
- * the history is marked (max_steps = MARK_LOOP_LIMIT) so that it ends at
- * its next step boundary with rc HISTORY_LOOP_LIMIT and no event.
+ * per event around 1e10 GeV, and at 1e10 GeV forward one of its loops never
+ * ends; a NaN or infinite input, or a sampling range that rounds to
+ * nothing, can also make a loop never accept. A single such lane keeps its
+ * wavefront, and so the launch, from ending: an iteration takes a few
+ * hundred cycles. After Config::loop_max iterations (2^16 by default) a
+ * loop gives up: its last values are used once and the history is marked
+ * (max_steps = MARK_LOOP_LIMIT) so that it ends at its next step boundary
+ * with rc HISTORY_LOOP_LIMIT and no event. Parity with the reference on
+ * histories above the tables takes a limit of 2^25.
  * Target selections that reach the reference's assert(0) mark the history
  * the same way (MARK_NO_TARGET, rc HISTORY_NO_TARGET).
  */
-#ifndef PG_LOOP_MAX
-#define PG_LOOP_MAX (1 << 25)
-#endif
-constexpr int LOOP_MAX = PG_LOOP_MAX;
+constexpr int LOOP_MAX_DEFAULT = 1 << 16;
 constexpr int MARK_LOOP_LIMIT = -1;
 constexpr int MARK_NO_TARGET = -2;
 
-PG_FN bool loop_stalled(Track & t, int & iterations)
+PG_FN bool loop_stalled(Track & t, int & iterations, int limit)
 {
-        if (++iterations < LOOP_MAX) return false;
+        if (++iterations < limit) return false;
         if (t.max_steps >= 0) t.max_steps = MARK_LOOP_LIMIT;
         return true;
 }
@@ -635,7 +636,7 @@ struct EngineT {
                                 do
                                         u = randn(t);
                                 while ((fabs(u) > 3.) &&
-                                    !loop_stalled(t, tries));
+                                    !loop_stalled(t, tries, c.loop_max));
                                 u /= 1.015387;
                                 k1 += u * dk1;
                         } else if (dk0 >= 1.7320508 * dk1) {
@@ -764,7 +765,7 @@ struct EngineT {
                                         s2 = -step;
                                         int halvings = 0;
                                         while ((fabs(s1 - s2) > STEP_MIN) &&
-                                            !loop_stalled(t, halvings)) {
+                                            !loop_stalled(t, halvings, c.loop_max)) {
                                                 const double s3 =
                                                     0.5 * (s1 + s2);
                                                 position[0] = pi[0] +
@@ -1114,7 +1115,7 @@ struct EngineT {
                         int tries = 0;
                         do {
                                 mu = ilb1 * bulk_neglog_uniform(t);
-                        } while ((mu > 1.) && !loop_stalled(t, tries));
+                        } while ((mu > 1.) && !loop_stalled(t, tries, c.loop_max));
                         rotate(t, mu);
                 }
                 density_io = locals_density;
@@ -1217,7 +1218,7 @@ struct EngineT {
                         for (;;) {
                                 zeta = t.rng.uniform() * stot;
                                 if ((zeta > 0.) && (zeta < stot)) break;
-                                if (loop_stalled(t, tries)) break;
+                                if (loop_stalled(t, tries, c.loop_max)) break;
                         }
                         /* The reference keeps the energy transfer and the
                          * weight in one union (struct del_info,
@@ -1283,7 +1284,7 @@ struct EngineT {
                         const double r0 = a0 + 1 / (q * q);
                         const double r1 = r0 - a1 / q;
                         if (t.rng.uniform() * r0 <= r1) break;
-                        if (loop_stalled(t, tries)) break;
+                        if (loop_stalled(t, tries, c.loop_max)) break;
                 }
                 return K - q;
         }
@@ -1404,7 +1405,7 @@ struct EngineT {
                         const double tmp = x0 + z * (x1 - x0);
                         r = 1. / tmp;
                         if ((r < xmin) || (r >= xmax)) {
-                                if (!loop_stalled(t, tries)) continue;
+                                if (!loop_stalled(t, tries, c.loop_max)) continue;
                                 w_bias = 0.;
                                 break;
                         }
@@ -1580,7 +1581,8 @@ struct EngineT {
                         const float zeta = t.rng.uniformf();
                         const float tmp = (1.f - zeta) + Af + zeta * mu0f;
                         if (tmp <= 0.f) {
-                                if (!loop_stalled(t, tries)) continue;
+                                if (!loop_stalled(t, tries, c.loop_max))
+                                        continue;
                                 mu1 = mu0f;
                                 break;
                         }
@@ -1599,7 +1601,7 @@ struct EngineT {
                         ratio *= ehs_form_factor_f32(mu1, N);
                         ratio *= ratio * (1.f - fspin * mu1);
                         if (t.rng.uniformf() <= ratio) break;
-                        if (loop_stalled(t, tries)) break;
+                        if (loop_stalled(t, tries, c.loop_max)) break;
                 }
                 return mu1;
         }
@@ -1687,7 +1689,8 @@ struct EngineT {
                         const double zeta = t.rng.uniform();
                         const double tmp = 1. + A - zeta * (1 - mu0);
                         if (tmp <= 0.) {
-                                if (!loop_stalled(t, tries)) continue;
+                                if (!loop_stalled(t, tries, c.loop_max))
+                                        continue;
                                 mu1 = mu0;
                                 break;
                         }
@@ -1708,7 +1711,7 @@ struct EngineT {
                             mu1, screening_at(d, n));
                         ratio *= ratio * (1. - d.fspin * mu1);
                         if (t.rng.uniform() <= ratio) break;
-                        if (loop_stalled(t, tries)) break;
+                        if (loop_stalled(t, tries, c.loop_max)) break;
                 }
                 }
 

@@ -1491,6 +1491,8 @@ static enum pumas_return transport_impl(const pumas_gpu_physics * physics,
         a.c.limit_grammage = context->limit.grammage;
         a.c.limit_time = context->limit.time;
         a.c.accuracy = context->accuracy;
+        a.c.loop_max = (options && options->max_loop > 0) ? options->max_loop :
+                                                            LOOP_MAX_DEFAULT;
         a.g = geometry->view;
         a.states = states;
         a.n = n;

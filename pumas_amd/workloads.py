@@ -67,6 +67,30 @@ def uniform(seed, stream, history, index):
     return ((w >> np.uint64(11)).astype(np.float64) + 0.5) * 2.0 ** -53
 
 
+def uniforms(seed, stream, history, count, chunk=1 << 20):
+    """Draws 0..count-1 of each history id: the same values as `uniform`,
+    with one Philox block per pair of draws, over chunks of ids."""
+    history = np.asarray(history, np.uint64)
+    out = [np.empty(history.shape, np.float64) for _ in range(count)]
+    for a in range(0, history.size, chunk):
+        h = history[a:a + chunk]
+        lo = (h & _MASK32).astype(np.uint32)
+        hi = (h >> np.uint64(32)).astype(np.uint32)
+        for b in range((count + 1) // 2):
+            x0, x1, x2, x3 = philox4x32_10(
+                np.full(h.shape, b, np.uint32),
+                np.full(h.shape, stream, np.uint32), lo, hi,
+                seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF)
+            for k, (xl, xh) in enumerate(((x0, x1), (x2, x3))):
+                if 2 * b + k >= count:
+                    break
+                w = (xh.astype(np.uint64) << np.uint64(32)) | \
+                    xl.astype(np.uint64)
+                out[2 * b + k][a:a + chunk] = \
+                    ((w >> np.uint64(11)).astype(np.float64) + 0.5) * 2.0 ** -53
+    return out
+
+
 def _ids(n, first):
     return np.arange(first, first + n, dtype=np.uint64)
 
@@ -129,9 +153,7 @@ def slabs_forward_states(n, seed=0, first=0, kmin=10.0, kmax=1e4,
     [kmin, kmax] GeV, cos(zenith) uniform in [cmin, cmax], azimuth uniform,
     going down, charge -1, weight 1."""
     ids = _ids(n, first)
-    u0 = uniform(seed, 0, ids, 0)
-    u1 = uniform(seed, 0, ids, 1)
-    u2 = uniform(seed, 0, ids, 2)
+    u0, u1, u2 = uniforms(seed, 0, ids, 3)
     s = np.zeros(n, STATE_DTYPE)
     s["charge"] = -1.0
     s["energy"] = kmin * np.exp(math.log(kmax / kmin) * u0)
@@ -162,9 +184,7 @@ def slabs_backward_states(n, seed=0, first=0, kmin=1e-1, kmax=1e3,
     in [kmin, kmax] GeV, momentum going down with cos(zenith) uniform in
     [cmin, cmax], weight 1/pdf, charge -1."""
     ids = _ids(n, first)
-    u0 = uniform(seed, 0, ids, 0)
-    u1 = uniform(seed, 0, ids, 1)
-    u2 = uniform(seed, 0, ids, 2)
+    u0, u1, u2 = uniforms(seed, 0, ids, 3)
     rk = math.log(kmax / kmin)
     s = np.zeros(n, STATE_DTYPE)
     s["charge"] = -1.0
@@ -285,7 +305,7 @@ def terrain_states(n, extent, seed=0, first=0, kmin=1e-2, kmax=1e2,
     the central half of the map at TERRAIN_DETECTOR_Z, momentum going down
     in a cone cos(zenith) in [cmin, cmax], K log-uniform, weight 1/pdf."""
     ids = _ids(n, first)
-    u = [uniform(seed, 0, ids, k) for k in range(5)]
+    u = uniforms(seed, 0, ids, 5)
     rk = math.log(kmax / kmin)
     s = np.zeros(n, STATE_DTYPE)
     s["charge"] = -1.0
@@ -368,7 +388,7 @@ def voxel_states(n, extent, seed=0, first=0, kmin=1e-1, kmax=1e3,
     """Final states at detectors near the bottom of the voxel cube, momentum
     going down in a cone, K log-uniform in [kmin, kmax] GeV."""
     ids = _ids(n, first)
-    u = [uniform(seed, 0, ids, k) for k in range(5)]
+    u = uniforms(seed, 0, ids, 5)
     rk = math.log(kmax / kmin)
     s = np.zeros(n, STATE_DTYPE)
     s["charge"] = -1.0

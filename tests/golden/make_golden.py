@@ -25,6 +25,17 @@ from pumas_amd import workloads as W  # noqa: E402
 
 SEED = 0
 
+# The reference's rejection loops have no bound. Above the last table row
+# the elastic hard scattering angle loop needs up to millions of trials, so
+# the overflow cases run with a loop limit of 2^25 iterations instead of the
+# engine's default 2^16.
+OVERFLOW_MAX_LOOP = 1 << 25
+
+
+def max_loop(name):
+    """The loop limit a fixture is checked with."""
+    return OVERFLOW_MAX_LOOP if name.startswith("overflow") else 1 << 16
+
 
 def edge_states(n_each, energies, grammage=0.0, distance=0.0, time=0.0,
                 direction=-1.0, z=0.0):
@@ -119,8 +130,8 @@ def cases():
            W.csda_rock_geometry(), edge_states(8, over, direction=1.0))
     # Stepping above K_max: up to 2e9 GeV (from ~1e10 GeV the reference's
     # elastic hard scattering angle loop needs up to millions of trials,
-    # beyond 2^25 at 3e10 GeV, where the engine ends the history with
-    # PUMAS_GPU_HISTORY_LOOP_LIMIT).
+    # beyond 2^25 at 3e10 GeV, where the engine, run with that limit, ends
+    # the history with PUMAS_GPU_HISTORY_LOOP_LIMIT).
     yield ("overflow_straggled_forward", "standard_c005",
            W.slabs_forward_context(), W.slabs_geometry(),
            edge_states(8, [2e9, 1.5e9, 1e9, 5e8], direction=-1.0,

@@ -69,9 +69,11 @@ extern "C" int host_load(const char *path) {
 
 static Recorder g_rec;
 static int g_max_steps = 10000000;
+static int g_loop_max = pg::LOOP_MAX_DEFAULT;
 
 /* Step cap of the following host_transport calls. */
 extern "C" void host_set_max_steps(int n) { g_max_steps = n; }
+extern "C" void host_set_loop_max(int n) { g_loop_max = n; }
 
 /* Frames of the following host_transport calls go to `frames` (NULL stops
  * recording), as the device recorder. */
@@ -101,7 +103,7 @@ extern "C" int host_transport(const pumas_gpu_context *ctx, const pumas_gpu_geom
     Config c; c.energy_loss = ctx->mode.energy_loss; c.decay = ctx->mode.decay; c.direction = ctx->mode.direction;
     c.scattering = ctx->mode.scattering; c.event = ctx->event; c.limit_energy = ctx->limit.energy;
     c.limit_distance = ctx->limit.distance; c.limit_grammage = ctx->limit.grammage; c.limit_time = ctx->limit.time;
-    c.accuracy = ctx->accuracy;
+    c.accuracy = ctx->accuracy; c.loop_max = g_loop_max;
     static FloatTables g_ft; Engine eng = {g_view, g_mv, c, g, g_ft, g_view, g_rec};
     for (int64_t i = 0; i < n; i++) {
         pumas_state *s = states + i; Walker w; Track &t = w.t;

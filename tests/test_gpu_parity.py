@@ -33,7 +33,7 @@ def test_golden_fixture(gpu_standard, name):
     g = load_golden(name)
     geom = pa.Geometry(gpu_standard, desc)
     r = pa.transport(gpu_standard, ctx, geom, g["initial"].copy(),
-                     seed=int(g["seed"]))
+                     seed=int(g["seed"]), max_loop=make_golden.max_loop(name))
     assert_parity(r, g, RTOL)
 
 

@@ -54,6 +54,8 @@ def build_host(name="libhost_engine.so", defines=()):
     lib.host_record.restype = None
     lib.host_set_max_steps.argtypes = [ctypes.c_int]
     lib.host_set_max_steps.restype = None
+    lib.host_set_loop_max.argtypes = [ctypes.c_int]
+    lib.host_set_loop_max.restype = None
     lib.host_transport.argtypes = [
         ctypes.POINTER(_abi.GpuContext), ctypes.POINTER(_abi.GeometryDesc),
         ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64,
@@ -66,8 +68,9 @@ def host():
     return build_host()
 
 
-def run_host(lib, physics, ctx, desc, states, seed=0):
+def run_host(lib, physics, ctx, desc, states, seed=0, max_loop=1 << 16):
     assert lib.host_load(pa.physics_path(physics).encode()) == 0
+    lib.host_set_loop_max(max_loop)
     s = states.copy()
     n = len(s)
     ev = np.zeros(n, np.int32)
@@ -102,7 +105,8 @@ def test_golden_cases_bit_exact(host, ref_standard, case):
     name, pname, ctx, desc, states = next(
         c for c in make_golden.cases() if c[0] == case)
     states = states[:1024]
-    out = run_host(host, pname, ctx, desc, states)
+    out = run_host(host, pname, ctx, desc, states,
+                   max_loop=make_golden.max_loop(name))
     ref = orc.transport(ref_standard, ctx,
                         orc.RefGeometry(desc), states, seed=0,
                         threads=8)
@@ -268,17 +272,15 @@ def test_step_cap_status(host):
     np.testing.assert_array_equal(ev[~long], ok[1][~long])
 
 
-def test_loop_limit_status():
-    """With the loop limit forced down to 2 iterations (a build of the
-    engine for this test), histories whose rejection loops give up end with
-    rc PUMAS_GPU_HISTORY_LOOP_LIMIT and no event; the histories whose loops
-    never reached the limit are bit identical to the normal build."""
-    normal = build_host()
-    forced = build_host("libhost_engine_loop2.so", ("PG_LOOP_MAX=2",))
+def test_loop_limit_status(host):
+    """With the loop limit set to 2 iterations, histories whose rejection
+    loops give up end with rc PUMAS_GPU_HISTORY_LOOP_LIMIT and no event; the
+    histories whose loops never reached the limit are bit identical to a run
+    with the default limit."""
     ctx, desc = W.slabs_forward_context(), W.slabs_geometry()
     states = W.slabs_forward_states(128, seed=3)
-    ok = run_host(normal, "standard_c005", ctx, desc, states)
-    out = run_host(forced, "standard_c005", ctx, desc, states)
+    ok = run_host(host, "standard_c005", ctx, desc, states)
+    out = run_host(host, "standard_c005", ctx, desc, states, max_loop=2)
     flagged = out[5] == int(pa.HistoryStatus.LOOP_LIMIT)
     assert flagged.any() and (~flagged).any()
     assert (out[1][flagged] == int(pa.Event.NONE)).all()
@@ -290,11 +292,12 @@ def test_energy_above_tables_loop_limit(host):
     """Forward straggled transport at 1e11 GeV, a hundred times the last
     table row: the elastic hard scattering angle loop of the reference
     accepts so rarely there (the nuclear form factor cuts the large angles)
-    that it needs more than 2^25 trials. The engine ends the history after
-    that many with PUMAS_GPU_HISTORY_LOOP_LIMIT and no event."""
+    that it needs more than 2^25 trials. Run with a loop limit of 2^25, the
+    engine ends the history after that many with
+    PUMAS_GPU_HISTORY_LOOP_LIMIT and no event."""
     import make_golden as M
     s = M.edge_states(2, [1e11], direction=-1.0, z=W.SLAB_START_Z)
     out = run_host(host, "standard_c005", W.slabs_forward_context(),
-                   W.slabs_geometry(), s)
+                   W.slabs_geometry(), s, max_loop=1 << 25)
     assert (out[5] == int(pa.HistoryStatus.LOOP_LIMIT)).all()
     assert (out[1] == int(pa.Event.NONE)).all()
