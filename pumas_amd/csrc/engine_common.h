@@ -466,6 +466,87 @@ PG_FN double interpolate(const V & p, int * last, A tx, B ty, C tm, double x)
         return pchip(t, y0, y1, m1, m2);
 }
 
+/*
+ * Lookups without a cached row (a history's first lookups, in the closed
+ * form CSDA of the entry kernel). They return the row table_index returns
+ * (the bracketing row of a non decreasing table is unique), but the loads
+ * of each round are independent of one another: a lookup costs two or
+ * three round trips to memory instead of the bisection's chain of
+ * log2(n) dependent loads.
+ */
+
+/* Row of table[0] <= value < table[n - 1]: rounds of 15 pivots loaded
+ * together, down to at most 16 rows, then those rows loaded together. The
+ * rows with table[j] <= value are a prefix of the table, so the row is the
+ * last pivot at or below the value, and in the end `lo` plus the count of
+ * such rows. */
+template <class C>
+PG_FN int table_row_wide(C table, int n, double value)
+{
+        int lo = 0, hi = n - 1; /* table[lo] <= value < table[hi] */
+        while (hi - lo > 16) {
+                const int span = hi - lo;
+                int nlo = lo, nhi = hi;
+#pragma unroll
+                for (int k = 1; k < 16; k++) {
+                        const int j = lo + (span * k) / 16;
+                        if (table[j] <= value) {
+                                if (j > nlo) nlo = j;
+                        } else if (j < nhi) {
+                                nhi = j;
+                        }
+                }
+                lo = nlo;
+                hi = nhi;
+        }
+        int count = 0;
+#pragma unroll
+        for (int k = 1; k < 16; k++) {
+                /* table[hi] > value: the rows past the range count 0. */
+                const int j = (lo + k < hi) ? lo + k : hi;
+                count += (table[j] <= value) ? 1 : 0;
+        }
+        return lo + count;
+}
+
+/* Row of K[0] <= x < K[n - 1] on the kinetic energy grid: the coarse index
+ * with its four candidate rows loaded together (k_index_fast), else the
+ * wide search. */
+template <class V>
+PG_FN int k_row(const V & p, double x)
+{
+        if constexpr (V::coarse_index) {
+                const int b = k_bin(x) - p.k_bin0;
+                if ((p.k_coarse != nullptr) && (b >= 0) && (b < p.k_nbins)) {
+                        const int i = p.k_coarse[b];
+                        const int last = p.n_energies - 1;
+                        int count = 0;
+#pragma unroll
+                        for (int k = 1; k <= 4; k++) {
+                                const int j = (i + k < last) ? i + k : last;
+                                count += (p.K[j] <= x) ? 1 : 0;
+                        }
+                        if (count < 4) return i + count;
+                }
+        }
+        return table_row_wide(p.K, p.n_energies, x);
+}
+
+/* The interpolation of `interpolate` on a known row i1: the same
+ * operations, so the same result. */
+template <class A, class B, class C>
+PG_FN double interpolate_row(int i1, A tx, B ty, C tm, double x)
+{
+        const double x0 = tx[i1], x1 = tx[i1 + 1];
+        const double y0 = ty[i1], y1 = ty[i1 + 1];
+        const double d0 = tm[i1], d1 = tm[i1 + 1];
+        const double dX = x1 - x0;
+        const double t = (x - x0) / dX;
+        const double m1 = d0 * dX;
+        const double m2 = (i1 + 1 > 1) ? d1 * dX : m1;
+        return pchip(t, y0, y1, m1, m2);
+}
+
 /* cel_grammage, src/pumas.c:3644-3669. */
 template <class V>
 PG_FN double cel_grammage(const V & p, Cache & ch, int scheme, int material, double kinetic)

@@ -376,14 +376,52 @@ struct EngineT {
 
         /* transport_with_csda, src/pumas.c:4777-4949 (no magnetic field).
          * Inlined into the entry kernel, its only caller. */
+        /* cel_kinetic_energy in CSDA for a lookup without a cached row;
+         * *row is the range table's row that brackets the grammage, or -1
+         * outside the table. */
+        PG_FN double csda_kinetic_energy(
+            Track & t, int material, double grammage, int * row) const
+        {
+                const int imax = p.n_energies - 1;
+                const auto X = tab2(p, p.X, MODE_CSDA, material, 0);
+                if ((grammage >= X[0]) && (grammage < X[imax])) {
+                        const int j = table_row_wide(X, p.n_energies, grammage);
+                        *row = j;
+                        return interpolate_row(j, X, p.K,
+                            tab2(p, p.K_dX, MODE_CSDA, material, 0), grammage);
+                }
+                *row = -1;
+                return cel_kinetic_energy(p, t.ix, MODE_CSDA, material, grammage);
+        }
+
         PG_FN int transport_with_csda(
             Track & t, int material, double density) const
         {
                 const double ki = t.energy;
                 const double di = t.distance;
                 const double ti = t.time;
-                const double xi = cel_grammage(p, t.ix, MODE_CSDA, material, ki);
-                const double Ti = cel_proper_time(p, t.ix, MODE_CSDA, material, ki);
+                /* The grammage, proper time and energy loss at ki, from
+                 * one row lookup (the loss is the reference's last lookup;
+                 * the order of these pure lookups does not matter). */
+                double xi, Ti, dEi;
+                const int imax = p.n_energies - 1;
+                if ((ki >= p.K[0]) && (ki < p.K[imax])) {
+                        const int i = k_row(p, ki);
+                        xi = interpolate_row(i, p.K,
+                            tab2(p, p.X, MODE_CSDA, material, 0),
+                            tab2(p, p.X_dK, MODE_CSDA, material, 0), ki);
+                        Ti = interpolate_row(i, p.K,
+                            tab2(p, p.T, MODE_CSDA, material, 0),
+                            tab2(p, p.T_dK, MODE_CSDA, material, 0), ki);
+                        dEi = interpolate_row(i, p.K,
+                            tab2(p, p.dE, MODE_CSDA, material, 0),
+                            tab2(p, p.dE_dK, MODE_CSDA, material, 0), ki);
+                } else {
+                        xi = cel_grammage(p, t.ix, MODE_CSDA, material, ki);
+                        Ti = cel_proper_time(p, t.ix, MODE_CSDA, material, ki);
+                        dEi = cel_energy_loss(
+                            p, t.ix, MODE_CSDA, material, ki);
+                }
 
                 int event = EVENT_NONE;
                 double xB;
@@ -429,15 +467,19 @@ struct EngineT {
                 if (xB <= 0.) return event;
 
                 double xf, kf;
+                /* Row of kf on the kinetic energy grid when kf comes from
+                 * the range table's row (the same row brackets it there,
+                 * up to rounding), else -1. */
+                int kf_row = -1;
                 if (c.direction == MODE_FORWARD) {
                         if (xi > xB) {
                                 xf = xi - xB;
                                 if (xB < EPSILON_X * xi) {
-                                        kf = ki - cel_energy_loss(p, t.ix, MODE_CSDA,
-                                                      material, ki) * xB;
+                                        kf = ki - dEi * xB;
                                         if (kf < 0.) kf = 0.;
                                 } else {
-                                        kf = cel_kinetic_energy(p, t.ix, MODE_CSDA, material, xf);
+                                        kf = csda_kinetic_energy(
+                                            t, material, xf, &kf_row);
                                 }
                         } else {
                                 xf = kf = 0.;
@@ -448,6 +490,7 @@ struct EngineT {
                                         return EVENT_LIMIT_ENERGY;
                                 if (kf < c.limit_energy) {
                                         kf = c.limit_energy;
+                                        kf_row = -1;
                                         xf = cel_grammage(p, t.ix, MODE_CSDA, material, kf);
                                         event = EVENT_LIMIT_ENERGY;
                                 }
@@ -459,10 +502,10 @@ struct EngineT {
                         } else {
                                 xf = xB + xi;
                                 if (xB < EPSILON_X * xi) {
-                                        kf = ki + cel_energy_loss(p, t.ix, MODE_CSDA,
-                                                      material, ki) * xB;
+                                        kf = ki + dEi * xB;
                                 } else {
-                                        kf = cel_kinetic_energy(p, t.ix, MODE_CSDA, material, xf);
+                                        kf = csda_kinetic_energy(
+                                            t, material, xf, &kf_row);
                                 }
                         }
                         if (c.event & EVENT_LIMIT_ENERGY) {
@@ -470,11 +513,19 @@ struct EngineT {
                                         return EVENT_LIMIT_ENERGY;
                                 if (kf > c.limit_energy) {
                                         kf = c.limit_energy;
+                                        kf_row = -1;
                                         xf = cel_grammage(p, t.ix, MODE_CSDA, material, kf);
                                         event = EVENT_LIMIT_ENERGY;
                                 }
                         }
                 }
+                /* The row of kf: checked against the grid, found afresh
+                 * if it is not the range table's row. */
+                if ((kf_row >= 0) &&
+                    !((kf >= p.K[kf_row]) && (kf < p.K[kf_row + 1])))
+                        kf_row = -1;
+                if ((kf_row < 0) && (kf >= p.K[0]) && (kf < p.K[imax]))
+                        kf_row = k_row(p, kf);
 
                 const double distance = fabs(xf - xi) / density;
                 t.energy = kf;
@@ -490,13 +541,22 @@ struct EngineT {
                         t.time = time_max;
                         t.decayed = decayed;
                         if (decayed) event = EVENT_VERTEX_DECAY;
-                } else
-                        t.time += fabs(Ti - cel_proper_time(p, t.ix, MODE_CSDA,
-                                                material, kf)) / density;
-                if (c.direction == MODE_BACKWARD)
-                        t.weight *=
-                            cel_energy_loss(p, t.ix, MODE_CSDA, material, kf) /
-                            cel_energy_loss(p, t.ix, MODE_CSDA, material, ki);
+                } else {
+                        const double Tf = (kf_row >= 0) ?
+                            interpolate_row(kf_row, p.K,
+                                tab2(p, p.T, MODE_CSDA, material, 0),
+                                tab2(p, p.T_dK, MODE_CSDA, material, 0), kf) :
+                            cel_proper_time(p, t.ix, MODE_CSDA, material, kf);
+                        t.time += fabs(Ti - Tf) / density;
+                }
+                if (c.direction == MODE_BACKWARD) {
+                        const double dEf = (kf_row >= 0) ?
+                            interpolate_row(kf_row, p.K,
+                                tab2(p, p.dE, MODE_CSDA, material, 0),
+                                tab2(p, p.dE_dK, MODE_CSDA, material, 0), kf) :
+                            cel_energy_loss(p, t.ix, MODE_CSDA, material, kf);
+                        t.weight *= dEf / dEi;
+                }
                 if (c.decay == MODE_WEIGHTED)
                         t.weight *= lm::exp(-fabs(ti - t.time) / p.ctau);
 

@@ -898,7 +898,8 @@ __device__ __forceinline__ void steps_flush(
 constexpr int INIT_BLOCK = 256;
 
 template <bool F32, bool CLOSED>
-__global__ void __launch_bounds__(INIT_BLOCK) k_init(KernelArgs args)
+__global__ void __launch_bounds__(INIT_BLOCK)
+    __attribute__((amdgpu_waves_per_eu(CLOSED ? 5 : 1))) k_init(KernelArgs args)
 {
         KERNEL_ARGS(args);
         __shared__ struct pumas_state stage[INIT_BLOCK];
@@ -911,8 +912,26 @@ __global__ void __launch_bounds__(INIT_BLOCK) k_init(KernelArgs args)
             (sizeof(struct pumas_state) / sizeof(double)));
         double * const lds = reinterpret_cast<double *>(stage);
         double * const hbm = reinterpret_cast<double *>(a.states + base);
-        for (uint32_t k = threadIdx.x; k < words; k += INIT_BLOCK)
-                lds[k] = hbm[k];
+        constexpr uint32_t WORDS =
+            INIT_BLOCK * (sizeof(struct pumas_state) / sizeof(double));
+        constexpr uint32_t PER_LANE = WORDS / INIT_BLOCK;
+        static_assert(sizeof(struct pumas_state) % sizeof(double) == 0,
+            "states are staged as 8 B words");
+        if (words == WORDS) {
+                /* A full block: every load of the lane issued before the
+                 * first wait (a loop with a runtime bound waits on each
+                 * load in turn, one HBM round trip per word). */
+                double v[PER_LANE];
+#pragma unroll
+                for (uint32_t k = 0; k < PER_LANE; k++)
+                        v[k] = hbm[threadIdx.x + k * INIT_BLOCK];
+#pragma unroll
+                for (uint32_t k = 0; k < PER_LANE; k++)
+                        lds[threadIdx.x + k * INIT_BLOCK] = v[k];
+        } else {
+                for (uint32_t k = threadIdx.x; k < words; k += INIT_BLOCK)
+                        lds[k] = hbm[k];
+        }
         __syncthreads();
         Walker w;
         w.phase = PHASE_EMPTY;
@@ -924,8 +943,18 @@ __global__ void __launch_bounds__(INIT_BLOCK) k_init(KernelArgs args)
                 if (!CLOSED && (w.phase == PHASE_STEP)) a.records[i] = w;
         }
         __syncthreads();
-        for (uint32_t k = threadIdx.x; k < words; k += INIT_BLOCK)
-                hbm[k] = lds[k];
+        if (words == WORDS) {
+                double v[PER_LANE];
+#pragma unroll
+                for (uint32_t k = 0; k < PER_LANE; k++)
+                        v[k] = lds[threadIdx.x + k * INIT_BLOCK];
+#pragma unroll
+                for (uint32_t k = 0; k < PER_LANE; k++)
+                        hbm[threadIdx.x + k * INIT_BLOCK] = v[k];
+        } else {
+                for (uint32_t k = threadIdx.x; k < words; k += INIT_BLOCK)
+                        hbm[k] = lds[k];
+        }
         if (!CLOSED)
                 queue_append(a.counters + C_STEP_N0, a.q_step[0],
                     w.phase == PHASE_STEP, (uint32_t)i);

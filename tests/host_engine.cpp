@@ -170,6 +170,21 @@ extern "C" void host_functions(int which, int ip, int iel, const double *a,
     }
 }
 
+/* Row lookups without a cached row: table_row_wide on table[0..n) (which
+ * 0), k_row on the loaded kinetic energy grid (which 1). */
+extern "C" void host_rows(int which, const double *table, int n,
+    const double *x, int *out, int64_t m) {
+    for (int64_t i = 0; i < m; i++)
+        out[i] = which ? k_row(g_view, x[i]) : table_row_wide(table, n, x[i]);
+}
+
+/* Copies the loaded kinetic energy grid into out[cap]; returns its size. */
+extern "C" int host_k_grid(double *out, int cap) {
+    for (int i = 0; (i < g_view.n_energies) && (i < cap); i++)
+        out[i] = g_view.K[i];
+    return g_view.n_energies;
+}
+
 /* The table accessors on the loaded tables, numbered as the oracle's
  * ref_cel: 0 cel_grammage, 1 cel_proper_time, 2 cel_kinetic_energy,
  * 3 cel_energy_loss, 4 cel_straggling, 5 cel_grammage_as_time,

@@ -99,3 +99,59 @@ def n_elements(physics):
     import struct
     data = open(pa.physics_path(physics), "rb").read(16 + 4 * 4)
     return struct.unpack_from("<4i", data, 16)[3]
+
+
+def _row_probes(table, rng):
+    """Values in [table[0], table[-1]): every entry, its neighbours in
+    floating point, the midpoints of the rows and random values."""
+    t = np.asarray(table, float)
+    x = np.concatenate([t, np.nextafter(t, np.inf), np.nextafter(t, -np.inf),
+                        0.5 * (t[:-1] + t[1:]),
+                        rng.uniform(t[0], t[-1], 4096)])
+    return x[(x >= t[0]) & (x < t[-1])]
+
+
+def test_fresh_row_lookups(host):
+    """The row lookups of a history without cached rows (closed form CSDA)
+    return the bracketing row: the largest i with table[i] <= x, computed
+    here with numpy's sorted search, on increasing tables of 2 to 1000 rows,
+    tables with repeated values, and the kinetic energy grid of both
+    physics (whose lookups go through the coarse index)."""
+    host.host_rows.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                               ctypes.c_void_p, ctypes.c_void_p,
+                               ctypes.c_int64]
+    host.host_rows.restype = None
+    host.host_k_grid.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    host.host_k_grid.restype = ctypes.c_int
+    rng = np.random.default_rng(5)
+
+    def rows(which, table, x):
+        table = np.ascontiguousarray(table, float)
+        out = np.full(len(x), -7, np.int32)
+        host.host_rows(which, table.ctypes.data, len(table), x.ctypes.data,
+                       out.ctypes.data, len(x))
+        return out
+
+    tables = [np.cumsum(rng.uniform(0.01, 1.0, n))
+              for n in (2, 3, 15, 16, 17, 18, 33, 146, 257, 300, 1000)]
+    plateau = np.cumsum(rng.uniform(0.01, 1.0, 146))
+    plateau[40:60] = plateau[40]
+    plateau[100:103] = plateau[100]
+    tables += [plateau, np.logspace(-3, 9, 146)]
+    for t in tables:
+        x = _row_probes(t, rng)
+        want = np.searchsorted(t, x, side="right") - 1
+        np.testing.assert_array_equal(rows(0, t, x), want,
+                                      err_msg=f"{len(t)} rows")
+    for physics in ("standard_c005", "iron_c001"):
+        assert host.host_load(pa.physics_path(physics).encode()) == 0
+        K = np.zeros(4096)
+        n = host.host_k_grid(K.ctypes.data, len(K))
+        K = K[:n]
+        x = _row_probes(K, rng)
+        x = np.concatenate([x, np.exp(rng.uniform(np.log(K[1]),
+                                                  np.log(K[-1]), 20000))])
+        x = x[(x >= K[0]) & (x < K[-1])]
+        want = np.searchsorted(K, x, side="right") - 1
+        np.testing.assert_array_equal(rows(1, K, x), want, err_msg=physics)
+
