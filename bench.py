@@ -592,8 +592,8 @@ def main():
                 "step measured) and is bound by VALU issue over divergent "
                 "lanes (VALUBusy ~66 %, VALUUtilization ~35 %); the closed "
                 "form entry kernel moves exactly its 208 B per history and "
-                "is latency bound (VALUBusy ~25 %); profiles/ holds the "
-                "counters"),
+                "is bound by the latency of its dependent table lookups, all "
+                "L2 hits (VALUBusy ~32 %); profiles/ holds the counters"),
         },
         "transport_ms_per_pass": mean_ms,
         "history_status": {

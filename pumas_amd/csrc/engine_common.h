@@ -475,37 +475,39 @@ PG_FN double interpolate(const V & p, int * last, A tx, B ty, C tm, double x)
  * log2(n) dependent loads.
  */
 
-/* Row of table[0] <= value < table[n - 1]: rounds of 15 pivots loaded
- * together, down to at most 16 rows, then those rows loaded together. The
- * rows with table[j] <= value are a prefix of the table, so the row is the
- * last pivot at or below the value, and in the end `lo` plus the count of
- * such rows. */
-template <class C>
+/* Row of table[0] <= value < table[n - 1]: rounds of W - 1 pivots loaded
+ * together, down to at most W rows, then those rows loaded together. The
+ * rows with table[j] <= value are a prefix of the table, so the count of
+ * loaded rows at or below the value places it. */
+template <int W = 8, class C>
 PG_FN int table_row_wide(C table, int n, double value)
 {
         int lo = 0, hi = n - 1; /* table[lo] <= value < table[hi] */
-        while (hi - lo > 16) {
+        double v[W - 1];
+        while (hi - lo > W) {
+                /* Pivots lo + span k / W, k = 1..W-1: strictly between lo
+                 * and hi, increasing (span > W). */
                 const int span = hi - lo;
-                int nlo = lo, nhi = hi;
 #pragma unroll
-                for (int k = 1; k < 16; k++) {
-                        const int j = lo + (span * k) / 16;
-                        if (table[j] <= value) {
-                                if (j > nlo) nlo = j;
-                        } else if (j < nhi) {
-                                nhi = j;
-                        }
-                }
-                lo = nlo;
+                for (int k = 1; k < W; k++)
+                        v[k - 1] = table[lo + (span * k) / W];
+                int count = 0;
+#pragma unroll
+                for (int k = 1; k < W; k++)
+                        count += (v[k - 1] <= value) ? 1 : 0;
+                const int nhi = lo + (span * (count + 1)) / W;
+                lo += (span * count) / W;
                 hi = nhi;
+        }
+#pragma unroll
+        for (int k = 1; k < W; k++) {
+                /* table[hi] > value: the rows past the range count 0. */
+                v[k - 1] = table[(lo + k < hi) ? lo + k : hi];
         }
         int count = 0;
 #pragma unroll
-        for (int k = 1; k < 16; k++) {
-                /* table[hi] > value: the rows past the range count 0. */
-                const int j = (lo + k < hi) ? lo + k : hi;
-                count += (table[j] <= value) ? 1 : 0;
-        }
+        for (int k = 1; k < W; k++)
+                count += (v[k - 1] <= value) ? 1 : 0;
         return lo + count;
 }
 

@@ -374,8 +374,6 @@ struct EngineT {
                     cos_theta * direction[2] + st * (cp * u0z + sp * u1z);
         }
 
-        /* transport_with_csda, src/pumas.c:4777-4949 (no magnetic field).
-         * Inlined into the entry kernel, its only caller. */
         /* cel_kinetic_energy in CSDA for a lookup without a cached row;
          * *row is the range table's row that brackets the grammage, or -1
          * outside the table. */
@@ -394,6 +392,8 @@ struct EngineT {
                 return cel_kinetic_energy(p, t.ix, MODE_CSDA, material, grammage);
         }
 
+        /* transport_with_csda, src/pumas.c:4777-4949 (no magnetic field).
+         * Inlined into the entry kernel, its only caller. */
         PG_FN int transport_with_csda(
             Track & t, int material, double density) const
         {

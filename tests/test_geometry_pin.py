@@ -45,7 +45,9 @@ def geo():
         subprocess.check_call(
             ["gcc", "-O2", "-std=c99", "-fPIC", "-shared",
              "-ffp-contract=off", "-I", os.path.join(REPO, "include"), "-I",
-             os.path.join(REPO, "pumas_amd", "csrc"), "-o", LIB, src])
+             os.path.join(REPO, "pumas_amd", "csrc"), "-o",
+             f"{LIB}.{os.getpid()}.tmp", src])
+        os.replace(f"{LIB}.{os.getpid()}.tmp", LIB)
     lib = ctypes.CDLL(LIB)
     lib.geo_check_locate.argtypes = [ctypes.POINTER(_abi.GeometryDesc)] + \
         [ctypes.c_void_p] * 2 + [ctypes.c_double, ctypes.c_int64] + \

@@ -41,11 +41,15 @@ def build_host(name="libhost_engine.so", defines=()):
                     if f.endswith(".h")]
     if (not os.path.exists(lib_path)) or any(
             os.path.getmtime(d) > os.path.getmtime(lib_path) for d in deps):
+        # Built under a name of this process, then renamed: test workers
+        # that rebuild together never load a partly written library.
+        tmp = f"{lib_path}.{os.getpid()}.tmp"
         subprocess.check_call(
             ["g++", "-O2", "-std=c++17", "-fPIC", "-shared",
              "-ffp-contract=off", "-mfma"] + [f"-D{d}" for d in defines] +
             ["-I", csrc, "-I", os.path.join(REPO, "include"), "-o",
-             lib_path, src])
+             tmp, src])
+        os.replace(tmp, lib_path)
     lib = ctypes.CDLL(lib_path)
     lib.host_load.argtypes = [ctypes.c_char_p]
     lib.host_record.argtypes = [ctypes.c_void_p, ctypes.c_void_p,

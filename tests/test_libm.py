@@ -26,7 +26,9 @@ def lib():
             os.path.getmtime(d) > os.path.getmtime(LIB) for d in deps):
         subprocess.check_call(
             ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp",
-             "-ffp-contract=off", "-mfma", "-I", csrc, "-o", LIB, src])
+             "-ffp-contract=off", "-mfma", "-I", csrc, "-o",
+             f"{LIB}.{os.getpid()}.tmp", src])
+        os.replace(f"{LIB}.{os.getpid()}.tmp", LIB)
     lib = ctypes.CDLL(LIB)
     lib.check_float.restype = ctypes.c_long
     lib.check_float.argtypes = [ctypes.c_int, ctypes.c_long, ctypes.c_long]
