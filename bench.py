@@ -139,8 +139,8 @@ def cpu_model():
 
 def cpu_threads():
     """Threads of the CPU baseline: every core of the affinity mask, capped
-    by OMP_NUM_THREADS when it is set (a shared GPU box gives each GPU a
-    share of its host cores and states it there)."""
+    by OMP_NUM_THREADS when it is set (a host shared by several GPUs gives
+    each GPU a share of its cores and states it there)."""
     affinity = len(os.sched_getaffinity(0))
     share = os.environ.get("OMP_NUM_THREADS")
     threads = affinity

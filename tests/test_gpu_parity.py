@@ -4,7 +4,7 @@ The device engine uses the reference's arithmetic step by step, and its
 math functions (pumas_amd/csrc/libm.h) return the bits of the C library
 the reference is built against, including the float logf / expf of
 dcs_evaluate (src/pumas.c:11298-11346). The bar is therefore the FP64
-north-star tolerance on every history: events, media, step and random
+per-event tolerance on every history: events, media, step and random
 draw counts identical (no branch flips), and final energy, distance,
 grammage, time, weight, position and direction within 1e-12 relative for
 every history (assert_parity prints the measured fraction within 1e-12 and
