@@ -475,6 +475,15 @@ PG_FN double interpolate(const V & p, int * last, A tx, B ty, C tm, double x)
  * log2(n) dependent loads.
  */
 
+/* Rows i and i + 1 of a table, as one 16 B load (8 B aligned). */
+PG_FN void row_pair(const double * t, int i, double & a, double & b)
+{
+        double v[2];
+        __builtin_memcpy(v, t + i, sizeof(v));
+        a = v[0];
+        b = v[1];
+}
+
 /* Row of table[0] <= value < table[n - 1]: rounds of W - 1 pivots loaded
  * together, down to at most W rows, then those rows loaded together. The
  * rows with table[j] <= value are a prefix of the table, so the count of
@@ -499,12 +508,21 @@ PG_FN int table_row_wide(C table, int n, double value)
                 lo += (span * count) / W;
                 hi = nhi;
         }
-#pragma unroll
-        for (int k = 1; k < W; k++) {
-                /* table[hi] > value: the rows past the range count 0. */
-                v[k - 1] = table[(lo + k < hi) ? lo + k : hi];
-        }
+        /* The rows past hi count 0: table[j] >= table[hi] > value. */
         int count = 0;
+        if ((W % 2 == 0) && (lo + W <= n - 1)) {
+                /* Rows lo + 1 .. lo + W in 16 B pairs. */
+#pragma unroll
+                for (int k = 1; k < W; k += 2) {
+                        double a, b;
+                        row_pair(table, lo + k, a, b);
+                        count += ((a <= value) ? 1 : 0) + ((b <= value) ? 1 : 0);
+                }
+                return lo + count;
+        }
+#pragma unroll
+        for (int k = 1; k < W; k++)
+                v[k - 1] = table[(lo + k < hi) ? lo + k : hi];
 #pragma unroll
         for (int k = 1; k < W; k++)
                 count += (v[k - 1] <= value) ? 1 : 0;
@@ -523,10 +541,21 @@ PG_FN int k_row(const V & p, double x)
                         const int i = p.k_coarse[b];
                         const int last = p.n_energies - 1;
                         int count = 0;
+                        if (i + 4 <= last) {
+                                /* K[i + 1 .. i + 4] in two 16 B pairs. */
+                                double a, b1, c, d;
+                                row_pair(p.K, i + 1, a, b1);
+                                row_pair(p.K, i + 3, c, d);
+                                count = ((a <= x) ? 1 : 0) +
+                                    ((b1 <= x) ? 1 : 0) + ((c <= x) ? 1 : 0) +
+                                    ((d <= x) ? 1 : 0);
+                        } else {
 #pragma unroll
-                        for (int k = 1; k <= 4; k++) {
-                                const int j = (i + k < last) ? i + k : last;
-                                count += (p.K[j] <= x) ? 1 : 0;
+                                for (int k = 1; k <= 4; k++) {
+                                        const int j =
+                                            (i + k < last) ? i + k : last;
+                                        count += (p.K[j] <= x) ? 1 : 0;
+                                }
                         }
                         if (count < 4) return i + count;
                 }
@@ -536,12 +565,13 @@ PG_FN int k_row(const V & p, double x)
 
 /* The interpolation of `interpolate` on a known row i1: the same
  * operations, so the same result. */
-template <class A, class B, class C>
-PG_FN double interpolate_row(int i1, A tx, B ty, C tm, double x)
+PG_FN double interpolate_row(
+    int i1, const double * tx, const double * ty, const double * tm, double x)
 {
-        const double x0 = tx[i1], x1 = tx[i1 + 1];
-        const double y0 = ty[i1], y1 = ty[i1 + 1];
-        const double d0 = tm[i1], d1 = tm[i1 + 1];
+        double x0, x1, y0, y1, d0, d1;
+        row_pair(tx, i1, x0, x1);
+        row_pair(ty, i1, y0, y1);
+        row_pair(tm, i1, d0, d1);
         const double dX = x1 - x0;
         const double t = (x - x0) / dX;
         const double m1 = d0 * dX;

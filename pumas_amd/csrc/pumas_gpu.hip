@@ -897,6 +897,19 @@ __device__ __forceinline__ void steps_flush(
  * 104 B strides. */
 constexpr int INIT_BLOCK = 256;
 
+/* 16 B at an 8 B aligned address (global or LDS), by value. */
+__device__ __forceinline__ double2 load16(const double * p)
+{
+        double2 v;
+        __builtin_memcpy(&v, p, sizeof(v));
+        return v;
+}
+
+__device__ __forceinline__ void store16(double * p, double2 v)
+{
+        __builtin_memcpy(p, &v, sizeof(v));
+}
+
 template <bool F32, bool CLOSED>
 __global__ void __launch_bounds__(INIT_BLOCK)
     __attribute__((amdgpu_waves_per_eu(CLOSED ? 5 : 1))) k_init(KernelArgs args)
@@ -914,20 +927,33 @@ __global__ void __launch_bounds__(INIT_BLOCK)
         double * const hbm = reinterpret_cast<double *>(a.states + base);
         constexpr uint32_t WORDS =
             INIT_BLOCK * (sizeof(struct pumas_state) / sizeof(double));
-        constexpr uint32_t PER_LANE = WORDS / INIT_BLOCK;
+        /* A full block moves 16 B words (states are 8 B aligned; the
+         * 16 B loads and stores need no more): 6.5 per lane. */
+        constexpr uint32_t PAIRS = WORDS / 2;
+        constexpr uint32_t PER_LANE = (PAIRS + INIT_BLOCK - 1) / INIT_BLOCK;
         static_assert(sizeof(struct pumas_state) % sizeof(double) == 0,
             "states are staged as 8 B words");
+        static_assert(WORDS % 2 == 0, "a block holds whole 16 B words");
         if (words == WORDS) {
-                /* A full block: every load of the lane issued before the
-                 * first wait (a loop with a runtime bound waits on each
-                 * load in turn, one HBM round trip per word). */
-                double v[PER_LANE];
+                /* Every load of the lane issued before the first wait (a
+                 * loop with a runtime bound waits on each load in turn,
+                 * one HBM round trip per word). */
+                /* Words k < PER_LANE - 1 exist for every lane; the last
+                 * for the first PAIRS % INIT_BLOCK lanes. */
+                const bool tail = threadIdx.x + (PER_LANE - 1) * INIT_BLOCK <
+                    PAIRS;
+                double2 v[PER_LANE];
 #pragma unroll
-                for (uint32_t k = 0; k < PER_LANE; k++)
-                        v[k] = hbm[threadIdx.x + k * INIT_BLOCK];
+                for (uint32_t k = 0; k < PER_LANE - 1; k++)
+                        v[k] = load16(hbm + 2 * (threadIdx.x + k * INIT_BLOCK));
+                v[PER_LANE - 1] = tail ? load16(hbm + 2 * (threadIdx.x +
+                    (PER_LANE - 1) * INIT_BLOCK)) : v[0];
 #pragma unroll
-                for (uint32_t k = 0; k < PER_LANE; k++)
-                        lds[threadIdx.x + k * INIT_BLOCK] = v[k];
+                for (uint32_t k = 0; k < PER_LANE - 1; k++)
+                        store16(lds + 2 * (threadIdx.x + k * INIT_BLOCK), v[k]);
+                if (tail)
+                        store16(lds + 2 * (threadIdx.x +
+                            (PER_LANE - 1) * INIT_BLOCK), v[PER_LANE - 1]);
         } else {
                 for (uint32_t k = threadIdx.x; k < words; k += INIT_BLOCK)
                         lds[k] = hbm[k];
@@ -944,13 +970,20 @@ __global__ void __launch_bounds__(INIT_BLOCK)
         }
         __syncthreads();
         if (words == WORDS) {
-                double v[PER_LANE];
+                const bool tail = threadIdx.x + (PER_LANE - 1) * INIT_BLOCK <
+                    PAIRS;
+                double2 v[PER_LANE];
 #pragma unroll
-                for (uint32_t k = 0; k < PER_LANE; k++)
-                        v[k] = lds[threadIdx.x + k * INIT_BLOCK];
+                for (uint32_t k = 0; k < PER_LANE - 1; k++)
+                        v[k] = load16(lds + 2 * (threadIdx.x + k * INIT_BLOCK));
+                v[PER_LANE - 1] = tail ? load16(lds + 2 * (threadIdx.x +
+                    (PER_LANE - 1) * INIT_BLOCK)) : v[0];
 #pragma unroll
-                for (uint32_t k = 0; k < PER_LANE; k++)
-                        hbm[threadIdx.x + k * INIT_BLOCK] = v[k];
+                for (uint32_t k = 0; k < PER_LANE - 1; k++)
+                        store16(hbm + 2 * (threadIdx.x + k * INIT_BLOCK), v[k]);
+                if (tail)
+                        store16(hbm + 2 * (threadIdx.x +
+                            (PER_LANE - 1) * INIT_BLOCK), v[PER_LANE - 1]);
         } else {
                 for (uint32_t k = threadIdx.x; k < words; k += INIT_BLOCK)
                         hbm[k] = lds[k];
