@@ -489,9 +489,10 @@ PG_FN void row_pair(const double * t, int i, double & a, double & b)
  * rows with table[j] <= value are a prefix of the table, so the count of
  * loaded rows at or below the value places it. */
 template <int W = 8, class C>
-PG_FN int table_row_wide(C table, int n, double value)
+PG_FN int table_row_wide(C table, int n, double value, int lo = 0,
+    int hi = -1)
 {
-        int lo = 0, hi = n - 1; /* table[lo] <= value < table[hi] */
+        if (hi < 0) hi = n - 1; /* table[lo] <= value < table[hi] */
         double v[W - 1];
         while (hi - lo > W) {
                 /* Pivots lo + span k / W, k = 1..W-1: strictly between lo
@@ -527,6 +528,33 @@ PG_FN int table_row_wide(C table, int n, double value)
         for (int k = 1; k < W; k++)
                 count += (v[k - 1] <= value) ? 1 : 0;
         return lo + count;
+}
+
+/* table_row_wide for a table that is the same for every active lane of
+ * the wavefront: its first round, 15 pivots over the whole table, reads
+ * them with scalar loads (the tables are not written while a kernel runs),
+ * then the lanes go on in their own sixteenth of the table. */
+template <class C>
+PG_FN int table_row_uniform(C table, int n, double value)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+        typedef const __attribute__((address_space(4))) double cdouble;
+        cdouble * const t = (cdouble *)table;
+#else
+        const double * const t = table;
+#endif
+        const int span = n - 1;
+        int lo = 0, hi = span;
+        if (span > 16) {
+                /* Pivots strictly increasing: a valid bracket [lo, hi]. */
+                int count = 0;
+#pragma unroll
+                for (int k = 1; k < 16; k++)
+                        count += (t[(span * k) / 16] <= value) ? 1 : 0;
+                lo = (span * count) / 16;
+                hi = (span * (count + 1)) / 16;
+        }
+        return table_row_wide(table, n, value, lo, hi);
 }
 
 /* Row of K[0] <= x < K[n - 1] on the kinetic energy grid: the coarse index

@@ -383,7 +383,18 @@ struct EngineT {
                 const int imax = p.n_energies - 1;
                 const auto X = tab2(p, p.X, MODE_CSDA, material, 0);
                 if ((grammage >= X[0]) && (grammage < X[imax])) {
+#ifdef __HIP_DEVICE_COMPILE__
+                        /* One material for the wavefront (the usual case):
+                         * the search's first round from scalar loads. */
+                        const int m0 = __builtin_amdgcn_readfirstlane(material);
+                        const int j = __all(material == m0) ?
+                            table_row_uniform(
+                                tab2(p, p.X, MODE_CSDA, m0, 0), p.n_energies,
+                                grammage) :
+                            table_row_wide(X, p.n_energies, grammage);
+#else
                         const int j = table_row_wide(X, p.n_energies, grammage);
+#endif
                         *row = j;
                         return interpolate_row(j, X, p.K,
                             tab2(p, p.K_dX, MODE_CSDA, material, 0), grammage);

@@ -171,11 +171,14 @@ extern "C" void host_functions(int which, int ip, int iel, const double *a,
 }
 
 /* Row lookups without a cached row: table_row_wide on table[0..n) (which
- * 0), k_row on the loaded kinetic energy grid (which 1). */
+ * 0), k_row on the loaded kinetic energy grid (which 1), table_row_uniform
+ * on table[0..n) (which 2). */
 extern "C" void host_rows(int which, const double *table, int n,
     const double *x, int *out, int64_t m) {
     for (int64_t i = 0; i < m; i++)
-        out[i] = which ? k_row(g_view, x[i]) : table_row_wide(table, n, x[i]);
+        out[i] = (which == 1) ? k_row(g_view, x[i]) :
+            (which == 2) ? table_row_uniform(table, n, x[i]) :
+            table_row_wide(table, n, x[i]);
 }
 
 /* Copies the loaded kinetic energy grid into out[cap]; returns its size. */

@@ -116,7 +116,9 @@ def test_fresh_row_lookups(host):
     return the bracketing row: the largest i with table[i] <= x, computed
     here with numpy's sorted search, on increasing tables of 2 to 1000 rows,
     tables with repeated values, and the kinetic energy grid of both
-    physics (whose lookups go through the coarse index)."""
+    physics (whose lookups go through the coarse index). The search with a
+    first round of 15 pivots over the whole table (the device reads them
+    with scalar loads when a wavefront shares the table) is checked too."""
     host.host_rows.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                                ctypes.c_void_p, ctypes.c_void_p,
                                ctypes.c_int64]
@@ -143,6 +145,8 @@ def test_fresh_row_lookups(host):
         want = np.searchsorted(t, x, side="right") - 1
         np.testing.assert_array_equal(rows(0, t, x), want,
                                       err_msg=f"{len(t)} rows")
+        np.testing.assert_array_equal(rows(2, t, x), want,
+                                      err_msg=f"{len(t)} rows, uniform")
     for physics in ("standard_c005", "iron_c001"):
         assert host.host_load(pa.physics_path(physics).encode()) == 0
         K = np.zeros(4096)
