@@ -475,6 +475,19 @@ PG_FN double interpolate(const V & p, int * last, A tx, B ty, C tm, double x)
  * log2(n) dependent loads.
  */
 
+/* A table value read through the constant address space: a scalar load
+ * when the address is the same for every lane of the wavefront (the
+ * tables are not written while a kernel runs). */
+PG_FN double uload(const double * t)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+        typedef const __attribute__((address_space(4))) double cdouble;
+        return *(cdouble *)t;
+#else
+        return *t;
+#endif
+}
+
 /* Rows i and i + 1 of a table, as one 16 B load (8 B aligned). */
 PG_FN void row_pair(const double * t, int i, double & a, double & b)
 {

@@ -377,25 +377,29 @@ struct EngineT {
         /* cel_kinetic_energy in CSDA for a lookup without a cached row;
          * *row is the range table's row that brackets the grammage, or -1
          * outside the table. */
-        PG_FN double csda_kinetic_energy(
-            Track & t, int material, double grammage, int * row) const
+        PG_FN double csda_kinetic_energy(Track & t, int material,
+            double grammage, int * row, double * k0, double * k1) const
         {
                 const int imax = p.n_energies - 1;
                 const auto X = tab2(p, p.X, MODE_CSDA, material, 0);
-                if ((grammage >= X[0]) && (grammage < X[imax])) {
 #ifdef __HIP_DEVICE_COMPILE__
-                        /* One material for the wavefront (the usual case):
-                         * the search's first round from scalar loads. */
-                        const int m0 = __builtin_amdgcn_readfirstlane(material);
-                        const int j = __all(material == m0) ?
-                            table_row_uniform(
-                                tab2(p, p.X, MODE_CSDA, m0, 0), p.n_energies,
-                                grammage) :
-                            table_row_wide(X, p.n_energies, grammage);
+                /* One material for the wavefront (the usual case): the
+                 * bounds and the search's first round from scalar loads. */
+                const int m0 = __builtin_amdgcn_readfirstlane(material);
+                const bool uniform = __all(material == m0);
 #else
-                        const int j = table_row_wide(X, p.n_energies, grammage);
+                const int m0 = material;
+                const bool uniform = false;
 #endif
+                const auto Xu = tab2(p, p.X, MODE_CSDA, m0, 0);
+                const double x_lo = uniform ? uload(Xu) : X[0];
+                const double x_hi = uniform ? uload(Xu + imax) : X[imax];
+                if ((grammage >= x_lo) && (grammage < x_hi)) {
+                        const int j = uniform ?
+                            table_row_uniform(Xu, p.n_energies, grammage) :
+                            table_row_wide(X, p.n_energies, grammage);
                         *row = j;
+                        row_pair(p.K, j, *k0, *k1);
                         return interpolate_row(j, X, p.K,
                             tab2(p, p.K_dX, MODE_CSDA, material, 0), grammage);
                 }
@@ -416,7 +420,7 @@ struct EngineT {
                  * the order of these pure lookups does not matter). */
                 double xi, Ti, dEi;
                 const int imax = p.n_energies - 1;
-                if ((ki >= p.K[0]) && (ki < p.K[imax])) {
+                if ((ki >= uload(p.K)) && (ki < uload(p.K + imax))) {
                         const int i = k_row(p, ki);
                         xi = interpolate_row(i, p.K,
                             tab2(p, p.X, MODE_CSDA, material, 0),
@@ -482,6 +486,7 @@ struct EngineT {
                  * the range table's row (the same row brackets it there,
                  * up to rounding), else -1. */
                 int kf_row = -1;
+                double kf_lo = 0., kf_hi = 0.; /* K[kf_row], K[kf_row + 1] */
                 if (c.direction == MODE_FORWARD) {
                         if (xi > xB) {
                                 xf = xi - xB;
@@ -489,8 +494,8 @@ struct EngineT {
                                         kf = ki - dEi * xB;
                                         if (kf < 0.) kf = 0.;
                                 } else {
-                                        kf = csda_kinetic_energy(
-                                            t, material, xf, &kf_row);
+                                        kf = csda_kinetic_energy(t, material,
+                                            xf, &kf_row, &kf_lo, &kf_hi);
                                 }
                         } else {
                                 xf = kf = 0.;
@@ -515,8 +520,8 @@ struct EngineT {
                                 if (xB < EPSILON_X * xi) {
                                         kf = ki + dEi * xB;
                                 } else {
-                                        kf = csda_kinetic_energy(
-                                            t, material, xf, &kf_row);
+                                        kf = csda_kinetic_energy(t, material,
+                                            xf, &kf_row, &kf_lo, &kf_hi);
                                 }
                         }
                         if (c.event & EVENT_LIMIT_ENERGY) {
@@ -532,10 +537,10 @@ struct EngineT {
                 }
                 /* The row of kf: checked against the grid, found afresh
                  * if it is not the range table's row. */
-                if ((kf_row >= 0) &&
-                    !((kf >= p.K[kf_row]) && (kf < p.K[kf_row + 1])))
+                if ((kf_row >= 0) && !((kf >= kf_lo) && (kf < kf_hi)))
                         kf_row = -1;
-                if ((kf_row < 0) && (kf >= p.K[0]) && (kf < p.K[imax]))
+                if ((kf_row < 0) && (kf >= uload(p.K)) &&
+                    (kf < uload(p.K + imax)))
                         kf_row = k_row(p, kf);
 
                 const double distance = fabs(xf - xi) / density;
