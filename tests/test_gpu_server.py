@@ -56,6 +56,29 @@ def test_mix_equals_alone(gpu_standard, geometries):
           np.median([o.latency_ms for o in served]))
 
 
+def test_mix_equals_alone_full_sizes(gpu_standard, geometries):
+    """Requests of 10^3 to 10^6 muons under a 200 GB device budget, as in
+    the serving workload: each result equals the same request run alone."""
+    reqs = W.serving_requests(0.06, rate=200.0, seed=5, nmin=1e3, nmax=1e6)
+    assert len(reqs) >= 6
+    assert max(r.n for r in reqs) > 1e5
+    with pa.Server(hbm_budget=200 * 10**9, slots=4) as srv:
+        tickets = []
+        for r in reqs:
+            ctx, s = inputs(r)
+            tickets.append(srv.submit(gpu_standard, ctx, geometries[r.geometry],
+                                      s, first_history=r.first))
+        served = [srv.wait(t) for t in tickets]
+    for r, out in zip(reqs, served):
+        ctx, s = inputs(r)
+        alone = pa.transport(gpu_standard, ctx, geometries[r.geometry], s,
+                             first_history=r.first)
+        same(out.result, alone)
+        assert out.rc == 0
+    print("requests", len(reqs), "muons", sum(r.n for r in reqs),
+          "p50 latency %.1f ms" % np.median([o.latency_ms for o in served]))
+
+
 def test_budget_admission(gpu_standard, geometries):
     """A budget that holds one request at a time still completes every
     request, in order of submission, with identical results."""
