@@ -479,11 +479,12 @@ def main():
     mean_ms = sum(st.kernel_ms for st in passes) / len(passes)
     if csda:
         # One muon-step per history on the closed form path; the dominant
-        # kernel is k_init (one launch per chunk of 2^25 histories), timed
-        # with HIP events around each launch on its stream.
+        # kernel is k_init (one launch per chunk of up to 2^30 histories: a
+        # closed form launch keeps no per-history records), timed with HIP
+        # events around each launch on its stream.
         total_steps = args.steps * n
-        kernel, launches = "k_init", args.steps * ((n + (1 << 25) - 1) >>
-                                                   25)
+        kernel, launches = "k_init", args.steps * ((n + (1 << 30) - 1) >>
+                                                   30)
         kernel_ms = sum(st.init_kernel_ms for st in passes)
         unit_bytes = BYTES_PER_HISTORY_CSDA
     else:

@@ -1193,6 +1193,12 @@ namespace {
  * bytes of HBM (about 11 GB). */
 constexpr size_t MAX_CHUNK = size_t(1) << 25;
 
+/* Histories per chunk of a closed form launch, which keeps no records or
+ * queues and indexes its histories in 64 bits: bounded by the entry
+ * kernel's grid (2^30 lanes), so that a large batch is one launch and one
+ * counter read instead of one per 2^25 histories. */
+constexpr size_t MAX_CLOSED_CHUNK = size_t(1) << 30;
+
 /* Default number of rounds launched between two reads of the queue
  * lengths, and its maximum. */
 constexpr int ROUNDS_PER_CHECK = 16;
@@ -1669,8 +1675,9 @@ static enum pumas_return transport_impl(const pumas_gpu_physics * physics,
                 tail_del, lanes_per_wave, lds_bytes, 0, 0, stream,
                 (stats && !graph && !fixed) ? ev.marks.data() : nullptr,
                 (stats && !fixed) ? ev.init : nullptr, closed };
-        for (size_t base = 0; base < n; base += MAX_CHUNK) {
-                const size_t m = (n - base < MAX_CHUNK) ? n - base : MAX_CHUNK;
+        const size_t chunk = closed ? MAX_CLOSED_CHUNK : MAX_CHUNK;
+        for (size_t base = 0; base < n; base += chunk) {
+                const size_t m = (n - base < chunk) ? n - base : chunk;
                 rc = transport_chunk(
                     a, base, m, f32, L, &total_steps, &error, stats, fixed);
                 if (rc != PUMAS_RETURN_SUCCESS) break;

@@ -170,6 +170,26 @@ def test_batch_split_invariance(gpu_standard):
     assert (joined == whole.states).all()
 
 
+def test_closed_form_batch_above_stepping_chunk(gpu_standard):
+    """A closed form CSDA batch larger than the 2^25 histories of a stepping
+    chunk runs as one launch; it equals the same batch transported in two
+    calls split at 2^25."""
+    n0 = 1 << 25
+    s = W.csda_rock_states(n0 + 4096, seed=13)
+    geom = pa.Geometry(gpu_standard, W.csda_rock_geometry())
+    ctx = W.csda_rock_context()
+    whole = pa.transport(gpu_standard, ctx, geom, s.copy(), seed=13)
+    a = pa.transport(gpu_standard, ctx, geom, s[:n0].copy(), seed=13)
+    b = pa.transport(gpu_standard, ctx, geom, s[n0:].copy(), seed=13,
+                     first_history=n0)
+    assert whole.total_steps == 0
+    assert (np.concatenate([a.states, b.states]) == whole.states).all()
+    for k in ("events", "media", "steps", "draws", "rc"):
+        np.testing.assert_array_equal(
+            np.concatenate([getattr(a, k), getattr(b, k)]),
+            getattr(whole, k))
+
+
 @pytest.mark.parametrize("energy_loss", [pa.Mode.STRAGGLED, pa.Mode.MIXED])
 def test_graph_replays_equal_eager_launches(gpu_standard, energy_loss):
     """The rounds between two host checks are replayed as a captured HIP
